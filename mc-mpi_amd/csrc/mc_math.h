@@ -1,0 +1,327 @@
+/*
+ * mc_math.h -- correctly rounded FP64 natural log and exp.
+ *
+ * The transport step (reference: src/layer.cpp:137 `-log(h)` and
+ * src/layer.cpp:175 `expf(-sig_a * di)`, restated in FP64) needs log and exp
+ * whose results do not depend on which libm or GPU math library evaluates
+ * them.  The only such definition is correct rounding: these functions return
+ * the double nearest to the exact value (ties to even), so the GPU kernels and
+ * any correctly rounded CPU evaluation agree bit for bit.
+ *
+ * Method (own implementation; no libm code or coefficients are used):
+ *   - Tang's table-driven argument reduction (P. T. P. Tang, "Table-driven
+ *     implementation of the exponential function" / "... of the logarithm
+ *     function", ACM TOMS 15(2) 1989 and 16(4) 1990), with tables computed from
+ *     their definitions by gen_math_tables.py;
+ *   - Taylor polynomials with exact rational coefficients 1/n and 1/n!;
+ *   - double-double arithmetic (Dekker 1971; Knuth's TwoSum) with explicit
+ *     fma, so no step depends on compiler contraction;
+ *   - Ziv's strategy: a fast evaluation with relative error below 2^-72, a
+ *     rounding test against a 2^-67 bound, and, when the test cannot decide, an
+ *     all-double-double evaluation (relative error about 2^-100).  The result
+ *     can differ from the correctly rounded one only when the exact value lies
+ *     within about 2^-100 (relative) of a rounding boundary.
+ *
+ * The header compiles as HIP device code (hipcc) or as plain host C/C++ (gcc)
+ * for CPU-side tests; the arithmetic is the same IEEE-754 binary64 sequence in
+ * both.  Build with -ffp-contract=off.
+ */
+#ifndef MC_MATH_H
+#define MC_MATH_H
+
+#include <stdint.h>
+#include <string.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MC_FN __device__ __forceinline__ static
+#define MC_FN_COLD __device__ __noinline__ static
+#define MC_TAB __device__ static const
+#else
+#define MC_FN static inline
+#define MC_FN_COLD static
+#define MC_TAB static const
+#endif
+
+#include "mc_math_tables.h"
+
+/* ---------------------------------------------------------------- basics */
+
+MC_FN uint64_t mc_asu64(double x) {
+  uint64_t u;
+  memcpy(&u, &x, 8);
+  return u;
+}
+MC_FN double mc_asf64(uint64_t u) {
+  double x;
+  memcpy(&x, &u, 8);
+  return x;
+}
+MC_FN double mc_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+/* s + t = a + b exactly (Knuth TwoSum, no magnitude condition). */
+MC_FN void mc_two_sum(double a, double b, double *s, double *t) {
+  double sum = a + b;
+  double bb = sum - a;
+  *t = (a - (sum - bb)) + (b - bb);
+  *s = sum;
+}
+/* s + t = a + b exactly when |a| >= |b| or a == 0 (Dekker FastTwoSum). */
+MC_FN void mc_fast_two_sum(double a, double b, double *s, double *t) {
+  double sum = a + b;
+  *t = b - (sum - a);
+  *s = sum;
+}
+/* (rh, rl) ~ (ah + al) * (bh + bl) */
+MC_FN void mc_dd_mul(double ah, double al, double bh, double bl, double *rh, double *rl) {
+  double p = ah * bh;
+  double e = mc_fma(ah, bh, -p);
+  e = e + (ah * bl + al * bh);
+  mc_fast_two_sum(p, e, rh, rl);
+}
+/* (rh, rl) ~ (ah + al) + (bh + bl), accurate version */
+MC_FN void mc_dd_add(double ah, double al, double bh, double bl, double *rh, double *rl) {
+  double s, t;
+  mc_two_sum(ah, bh, &s, &t);
+  t = t + (al + bl);
+  mc_fast_two_sum(s, t, rh, rl);
+}
+
+/* 1 when hi + lo, known to within err (absolute), rounds to hi regardless of
+   where in [lo - err, lo + err] the exact tail lies. */
+MC_FN int mc_round_ok(double hi, double lo, double err) {
+  return (hi + (lo + err) == hi) && (hi + (lo - err) == hi);
+}
+
+/* ------------------------------------------------------------------- log */
+
+/* log(x) as double-double, all steps in double-double (about 2^-100 rel). */
+MC_FN_COLD double mc_log_accurate(int e, double r, int idx) {
+  /* log1p(r), |r| < 2^-8: Horner over 1/n in double-double for n <= 6, the
+     tail r^7/7 - ... + r^14/14 in double (below 2^-56 * r, error < 2^-108). */
+  double t = 0.0;
+  for (int n = 14; n >= 7; --n) {
+    double c = (n & 1) ? mc_inv_n[n][0] : -mc_inv_n[n][0];
+    t = mc_fma(t, r, c);
+  }
+  /* y = t*r + (-1/6); then Horner down to 1/1 in double-double */
+  double yh, yl;
+  {
+    double ph = t * r, pl = mc_fma(t, r, -ph);
+    mc_dd_add(ph, pl, -mc_inv_n[6][0], -mc_inv_n[6][1], &yh, &yl);
+  }
+  for (int n = 5; n >= 1; --n) {
+    double ph, pl;
+    mc_dd_mul(yh, yl, r, 0.0, &ph, &pl);
+    double ch = (n & 1) ? mc_inv_n[n][0] : -mc_inv_n[n][0];
+    double cl = (n & 1) ? mc_inv_n[n][1] : -mc_inv_n[n][1];
+    mc_dd_add(ph, pl, ch, cl, &yh, &yl);
+  }
+  double lh, ll; /* log1p(r) = y * r */
+  mc_dd_mul(yh, yl, r, 0.0, &lh, &ll);
+  /* + (-log c) */
+  double sh, sl;
+  mc_dd_add(lh, ll, mc_log_tab[idx][1], mc_log_tab[idx][2], &sh, &sl);
+  /* + e*ln2 (three-piece ln2; e*MC_LN2_HI is exact) */
+  double ed = (double)e;
+  double kh = ed * MC_LN2_HI;
+  double mh = ed * MC_LN2_MID, ml = mc_fma(ed, MC_LN2_MID, -mh);
+  ml = ml + ed * MC_LN2_LO;
+  double k2h, k2l;
+  mc_dd_add(kh, 0.0, mh, ml, &k2h, &k2l);
+  double rh, rl;
+  mc_dd_add(sh, sl, k2h, k2l, &rh, &rl);
+  return rh + rl;
+}
+
+MC_FN_COLD double mc_log_special(double x) {
+  uint64_t ix = mc_asu64(x);
+  if (x != x) return x + x;                       /* NaN */
+  if ((ix << 1) == 0) return -1.0 / 0.0;           /* +-0 -> -inf */
+  if (ix >> 63) return (x - x) / (x - x);          /* x < 0 -> NaN */
+  if (ix == 0x7FF0000000000000ULL) return x;       /* +inf */
+  return 0.0; /* subnormals are handled by the caller */
+}
+
+MC_FN double mc_log(double x) {
+  uint64_t ix = mc_asu64(x);
+  int eadj = 0;
+  /* x not a positive normal finite number */
+  if (ix - 0x0010000000000000ULL >= 0x7FF0000000000000ULL - 0x0010000000000000ULL) {
+    if (ix == 0 || (ix >> 63) || ix >= 0x7FF0000000000000ULL) return mc_log_special(x);
+    /* positive subnormal: scale into the normal range */
+    ix = mc_asu64(x * 0x1p54);
+    eadj = -54;
+  }
+  uint64_t tmp = ix - MC_LOG_OFF;
+  int e = (int)((int64_t)tmp >> 52);
+  double m = mc_asf64(ix - ((uint64_t)(int64_t)e << 52)); /* m in [sqrt(.5), sqrt(2)) */
+  e += eadj;
+  int idx = (int)__builtin_rint((m - 1.0) * MC_LOG_SCALE) - MC_LOG_IMIN;
+  double c = mc_log_tab[idx][0];
+  double r = mc_fma(m, c, -1.0); /* exact, |r| < 2^-8 */
+
+  /* log1p(r) = r - r^2/2 + r^3 * P(r), P = 1/3 - r/4 + ... + r^6/9 */
+  double r2h = r * r;
+  double r2l = mc_fma(r, r, -r2h);
+  double p = mc_fma(r, mc_fma(r, mc_inv_n[9][0], -mc_inv_n[8][0]), mc_inv_n[7][0]);
+  p = mc_fma(r, p, -mc_inv_n[6][0]);
+  p = mc_fma(r, p, mc_inv_n[5][0]);
+  p = mc_fma(r, p, -mc_inv_n[4][0]);
+  p = mc_fma(r, p, mc_inv_n[3][0]);
+  double t = (r2h * r) * p;
+
+  /* e*ln2 + (-log c) + r - r^2/2 + t, accumulated in double-double */
+  double ed = (double)e;
+  double s, serr, s2, serr2, s3, serr3;
+  mc_two_sum(ed * MC_LN2_HI, mc_log_tab[idx][1], &s, &serr);
+  double lo = serr + (ed * MC_LN2_MID + mc_log_tab[idx][2]);
+  mc_two_sum(s, r, &s2, &serr2);
+  lo = lo + serr2;
+  mc_two_sum(s2, -0.5 * r2h, &s3, &serr3);
+  lo = lo + (serr3 + (t - 0.5 * r2l));
+  double hi;
+  mc_fast_two_sum(s3, lo, &hi, &lo);
+  if (mc_round_ok(hi, lo, __builtin_fabs(hi) * 0x1p-67)) return hi + lo;
+  return mc_log_accurate(e, r, idx);
+}
+
+/* ------------------------------------------------------------------- exp */
+
+/* exp(xh + xl) as (yh, yl) * 2^k, all steps in double-double; |xh| < 800. */
+MC_FN_COLD void mc_exp_dd(double xh, double xl, double *yh, double *yl, int *kexp) {
+  double kd = __builtin_rint(xh * MC_INV_LN2_128);
+  int k = (int)kd;
+  /* r = x - k*ln2/128 with ln2/128 in three pieces */
+  double t = xh - kd * MC_LN2_128_1; /* exact */
+  double ph = kd * MC_LN2_128_2, pl = mc_fma(kd, MC_LN2_128_2, -ph);
+  double rh, rl;
+  mc_two_sum(t, -ph, &rh, &rl);
+  rl = rl + ((xl - pl) - kd * MC_LN2_128_3);
+  mc_fast_two_sum(rh, rl, &rh, &rl);
+  /* exp(r) = sum r^n/n!, |r| < 2^-8.4: n >= 6 in double, n <= 5 double-double */
+  double tail = 0.0;
+  for (int n = 13; n >= 6; --n) tail = mc_fma(tail, rh, mc_inv_fact[n][0]);
+  double eh, el;
+  {
+    double qh = tail * rh, ql = mc_fma(tail, rh, -qh);
+    mc_dd_add(qh, ql, mc_inv_fact[5][0], mc_inv_fact[5][1], &eh, &el);
+  }
+  for (int n = 4; n >= 0; --n) {
+    double qh, ql;
+    mc_dd_mul(eh, el, rh, rl, &qh, &ql);
+    mc_dd_add(qh, ql, mc_inv_fact[n][0], mc_inv_fact[n][1], &eh, &el);
+  }
+  int j = k & 127;
+  mc_dd_mul(mc_exp_tab[j][0], mc_exp_tab[j][1], eh, el, yh, yl);
+  *kexp = (k - j) / 128;
+}
+
+/* exp(x) for 2^-54 <= |x| < 2^-30.  Near 1 a double-double of the result
+   cannot hold cases such as exp(2^-53) = 1 + 2^-53 + 2^-107 + ..., which
+   lie 2^-107 above a rounding boundary.  Instead compute u = exp(x) - 1 =
+   x + x^2/2 + x^3/6 + x^4/24 in double-double relative to u (error below
+   2^-100 |u|, truncation below 2^-120 |u|) and round 1 + u on the grid of
+   doubles near 1 (spacing 2^-52 above 1, 2^-53 below). */
+MC_FN_COLD double mc_exp_tiny(double x) {
+  double x2h = x * x, x2l = mc_fma(x, x, -x2h);
+  double x3h = x2h * x, x3l = mc_fma(x2h, x, -x3h) + x2l * x;
+  double ch, cl;
+  mc_dd_mul(x3h, x3l, mc_inv_fact[3][0], mc_inv_fact[3][1], &ch, &cl);
+  double d4 = (x2h * x2h) * mc_inv_fact[4][0];
+  double uh, ul;
+  mc_dd_add(0.5 * x2h, 0.5 * x2l, ch, cl + d4, &uh, &ul);
+  mc_dd_add(x, 0.0, uh, ul, &uh, &ul);
+  /* v = |u| in units of the grid spacing, as vh + vl; n = round(v) */
+  int up = uh > 0.0;
+  double sc = up ? 0x1p52 : -0x1p53;
+  double vh = uh * sc, vl = ul * sc;
+  double n = __builtin_rint(vh);
+  double f = vh - n; /* exact, |f| <= 1/2 */
+  if (f - 0.5 > -vl) n += 1.0;        /* f + vl > 1/2 */
+  else if (f + 0.5 < -vl) n -= 1.0;   /* f + vl < -1/2 */
+  return up ? 1.0 + n * 0x1p-52 : 1.0 - n * 0x1p-53;
+}
+
+/* 2^n as a double for n in [-1022, 1023] */
+MC_FN double mc_pow2(int n) { return mc_asf64((uint64_t)(n + 1023) << 52); }
+
+MC_FN_COLD double mc_exp_special(double x) {
+  if (x != x) return x + x;          /* NaN */
+  if (x < -745.2) return 0.0;        /* exp(x) < 2^-1075: rounds to 0 (also -inf) */
+  if (x > 709.782712893384) return 1.0 / 0.0; /* overflow (also +inf) */
+  double yh, yl;
+  int k;
+  if (x > 0.0) {
+    /* 709.7 .. 709.78: result near DBL_MAX; scale in two steps */
+    mc_exp_dd(x, 0.0, &yh, &yl, &k);
+    double y = yh + yl;
+    return (y * mc_pow2(k - 1)) * 2.0;
+  }
+  /* subnormal range: exp(x) = z * 2^-1074 with z = exp(x + 1074 ln2);
+     round z to the nearest integer (ties to even). */
+  double th = 1074.0 * MC_LN2_HI; /* exact */
+  double mh = 1074.0 * MC_LN2_MID, ml = mc_fma(1074.0, MC_LN2_MID, -mh);
+  ml = ml + 1074.0 * MC_LN2_LO;
+  double sh, sl;
+  mc_two_sum(x, th, &sh, &sl);
+  double s2h, s2l;
+  mc_dd_add(sh, sl, mh, ml, &s2h, &s2l);
+  mc_exp_dd(s2h, s2l, &yh, &yl, &k);
+  /* z = (yh + yl) * 2^k, k <= 53, exact scaling */
+  double zs = mc_pow2(k);
+  double zh = yh * zs, zl = yl * zs;
+  double n = __builtin_rint(zh);
+  double f = (zh - n) + zl;
+  if (f > 0.5) n += 1.0;
+  else if (f < -0.5) n -= 1.0;
+  else if (f == 0.5 || f == -0.5) {
+    double nn = n + (f > 0 ? 1.0 : -1.0);
+    if (__builtin_fmod(nn, 2.0) == 0.0) n = nn;
+  }
+  return n * 0x1p-1074;
+}
+
+MC_FN double mc_exp(double x) {
+  if (!(x > -708.3 && x < 709.7)) return mc_exp_special(x);
+  /* |x| < 2^-54: exp(x) rounds to 1 (1 + x lies strictly inside the
+     rounding interval of 1, and so does exp(x) = 1 + x + O(x^2)). */
+  if (__builtin_fabs(x) < 0x1p-54) return 1.0;
+  double kd = __builtin_rint(x * MC_INV_LN2_128);
+  int k = (int)kd;
+  double t = x - kd * MC_LN2_128_1; /* exact */
+  double ph = kd * MC_LN2_128_2, pl = mc_fma(kd, MC_LN2_128_2, -ph);
+  double rh, rl;
+  mc_two_sum(t, -ph, &rh, &rl);
+  rl = rl - (pl + kd * MC_LN2_128_3);
+  mc_fast_two_sum(rh, rl, &rh, &rl);
+  /* exp(r) - 1 = r + r^2/2 + r^3 * Q(r), Q = 1/6 + r/24 + ... + r^4/5040 */
+  double r2h = rh * rh;
+  double r2l = mc_fma(rh, rh, -r2h);
+  double q = mc_fma(rh, mc_inv_fact[7][0], mc_inv_fact[6][0]);
+  q = mc_fma(rh, q, mc_inv_fact[5][0]);
+  q = mc_fma(rh, q, mc_inv_fact[4][0]);
+  q = mc_fma(rh, q, mc_inv_fact[3][0]);
+  q = (r2h * rh) * q;
+  double ah, al;
+  mc_fast_two_sum(rh, 0.5 * r2h, &ah, &al);
+  al = al + (rl + (0.5 * r2l + (rh * rl + q)));
+  /* 1 + a */
+  double eh, el;
+  mc_fast_two_sum(1.0, ah, &eh, &el);
+  el = el + al;
+  int j = k & 127;
+  double th = mc_exp_tab[j][0], tl = mc_exp_tab[j][1];
+  double yh = th * eh;
+  double yl = mc_fma(th, eh, -yh);
+  yl = yl + (th * el + tl * eh);
+  mc_fast_two_sum(yh, yl, &yh, &yl);
+  double sc = mc_pow2((k - j) / 128);
+  if (mc_round_ok(yh, yl, yh * 0x1p-67)) return (yh + yl) * sc;
+  if (__builtin_fabs(x) < 0x1p-30) return mc_exp_tiny(x);
+  int kk;
+  mc_exp_dd(x, 0.0, &yh, &yl, &kk);
+  return (yh + yl) * mc_pow2(kk);
+}
+
+#endif /* MC_MATH_H */
