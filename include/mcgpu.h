@@ -1,0 +1,174 @@
+/*
+ * mcgpu.h -- C ABI of the MI355X (gfx950) particle-transport library that
+ * replaces the hot path of mc-mpi's Layer::simulate.
+ *
+ * Reference interface replaced: the GPU branch of Layer::simulate
+ * (src/layer.cpp:254-300) calls
+ *
+ *   void cusimulate(int n, Particle *particles, float const *sigs,
+ *                   float const *absorption_rates, float *weights_absorbed,
+ *                   int min_index, int max_index, float dx);
+ *                                       (include/culayer/culayer.hpp:6-13)
+ *
+ * and the CPU branch runs the OpenMP loop of src/layer.cpp:303-330 over the
+ * same last-n particles.  mcgpu_simulate() is the drop-in for both: same
+ * argument meaning and order, FP64 instead of FP32, plus the minimum weight
+ * the CPU branch applies (src/layer.cpp:195) and an exit code per particle so
+ * the caller can fill particles_left / particles_right exactly as
+ * src/layer.cpp:332-346 does.
+ *
+ * Conventions
+ *  - Every entry point returns an int status (MCGPU_OK or a negative code) and
+ *    never terminates the process (the reference's gpu_errcheck calls exit(),
+ *    include/gpu_errcheck/gpu_errcheck.hpp:10-18).
+ *  - Per-particle outputs are indexed by array position p.  The reference
+ *    transports particles[size-1-i] in iteration i; its result[i] is our
+ *    result[n-1-i].
+ *  - weights_absorbed is accumulated (+=), like the reference.  Each call's
+ *    tally is an exact sum (fixed point, 2^-88 resolution), rounded once to
+ *    double and then added, so it is independent of thread scheduling.
+ *  - All arithmetic is IEEE binary64; log and exp are correctly rounded.
+ *  - Functions taking a `void *stream` accept a hipStream_t; NULL is HIP's
+ *    default (null) stream.  mcgpu_simulate (host buffers) uses the
+ *    context's own stream.  No torch or HIP types appear in signatures.
+ */
+#ifndef MCGPU_H
+#define MCGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------- status */
+enum {
+  MCGPU_OK = 0,
+  MCGPU_ERR_ARG = -1,       /* invalid argument (null pointer, bad size, ...) */
+  MCGPU_ERR_HIP = -2,       /* a HIP runtime call failed; see mcgpu_context_error */
+  MCGPU_ERR_NOMEM = -3,     /* device allocation failed */
+  MCGPU_ERR_NO_DEVICE = -4, /* no HIP device / bad device ordinal */
+  MCGPU_ERR_EVENT_CAP = -5, /* a history exceeded options.max_events_per_history */
+  MCGPU_ERR_RANGE = -6      /* a tally term outside +-2^39 (cell tally set to NaN) */
+};
+
+/* ------------------------------------------------------------- types */
+
+/* include/types/particle.hpp:7-17 with real_t = double.  40 bytes. */
+typedef struct mcgpu_particle {
+  uint64_t seed;  /* live RNG state of the history */
+  double x;
+  double mu;
+  double wmc;
+  int32_t index;  /* global cell index */
+  int32_t pad_;
+} mcgpu_particle;
+
+/* A particle bank in device memory, structure of arrays (n entries each). */
+typedef struct mcgpu_bank {
+  uint64_t *seed;
+  double *x;
+  double *mu;
+  double *wmc;
+  int32_t *index;
+} mcgpu_bank;
+
+enum {
+  MCGPU_KERNEL_HISTORY = 0, /* one history per lane, lanes refilled from a work queue */
+  MCGPU_KERNEL_EVENT = 1    /* one event per live particle per pass, SoA banks, compaction */
+};
+
+typedef struct mcgpu_options {
+  int32_t kernel;                  /* MCGPU_KERNEL_* */
+  int32_t block_size;              /* threads per block, 0 = automatic */
+  int32_t grid_blocks;             /* blocks, 0 = automatic */
+  int32_t events_per_refill;       /* history kernel: events between refills, 0 = automatic */
+  int32_t compaction_interval;     /* event kernel: passes between compactions, 0 = automatic */
+  int32_t reserved_;
+  uint64_t max_events_per_history; /* guard against runaway histories, 0 = 2^27 */
+} mcgpu_options;
+
+/* Device-counted totals of one call. */
+typedef struct mcgpu_stats {
+  uint64_t events;     /* particle_step calls (src/layer.cpp:123) */
+  uint64_t collisions; /* events that took the collision branch */
+  int64_t n_left;      /* histories that left through the low face */
+  int64_t n_right;     /* ... through the high face */
+  int64_t n_killed;    /* histories ended by the minimum weight */
+  double kernel_ms;    /* device time of the transport kernel(s) */
+} mcgpu_stats;
+
+typedef struct mcgpu_context mcgpu_context;
+
+/* ------------------------------------------------------------- context */
+
+/* Create a context on HIP device `device`: a stream of default priority and
+   grow-only device scratch buffers reused by later calls. */
+int mcgpu_context_create(int device, mcgpu_context **out);
+int mcgpu_context_destroy(mcgpu_context *ctx);
+const char *mcgpu_status_string(int status);
+/* Text of the last failure on this context ("" if none). */
+const char *mcgpu_context_error(const mcgpu_context *ctx);
+void mcgpu_default_options(mcgpu_options *opt);
+/* Library build identification, e.g. "mcgpu gfx950 ...". */
+const char *mcgpu_version(void);
+
+/* ------------------------------------------------------------- transport */
+
+/*
+ * Drop-in for cusimulate (src/culayer.cu:41-92) and the CPU loop of
+ * Layer::simulate (src/layer.cpp:303-330).
+ *   n                 number of particles; particles[0..n) are transported
+ *                     in place (host memory, in/out)
+ *   sigs, absorption_rates  host arrays of max_index - min_index cells
+ *   weights_absorbed  host array of max_index - min_index cells, +=
+ *   min_index, max_index  the layer's global cell range [min, max)
+ *   dx                the layer's cell width (Layer::dx)
+ *   particle_min_weight   Layer::particle_min_weight
+ *   result            host, n entries (may be NULL): exit code of
+ *                     particles[p]: -1 left face, +1 right face, 0 killed
+ *   opt, stats        may be NULL
+ */
+int mcgpu_simulate(mcgpu_context *ctx, int32_t n, mcgpu_particle *particles,
+                   const double *sigs, const double *absorption_rates,
+                   double *weights_absorbed, int32_t min_index, int32_t max_index, double dx,
+                   double particle_min_weight, int32_t *result, const mcgpu_options *opt,
+                   mcgpu_stats *stats);
+
+/*
+ * Same transport on a device-resident bank (for banks that live in HBM).
+ * All pointers are device pointers; result/events/collisions may be NULL.
+ * The call is stream-ordered on `stream` and returns after the transport
+ * finished (it reads back the device counters).
+ */
+int mcgpu_simulate_device(mcgpu_context *ctx, int64_t n, const mcgpu_bank *bank,
+                          const double *sigs, const double *absorption_rates,
+                          double *weights_absorbed, int32_t min_index, int32_t max_index,
+                          double dx, double particle_min_weight, int8_t *result,
+                          uint64_t *events, uint64_t *collisions, const mcgpu_options *opt,
+                          mcgpu_stats *stats, void *stream);
+
+/* ------------------------------------------------------------- RNG helpers
+ * Known-answer helpers for the reference RNG (src/random.cpp), evaluated on
+ * the device (all pointers are device pointers):
+ *   mcgpu_rng_real_step: seeds[i] <- one rnd_real step, reals[i] <- its value
+ *                        (the kernel of src/curandom.cu:7-14, in FP64);
+ *   mcgpu_rng_seed_chain: out[i] = the (i+1)-th rnd_seed output from `seed`,
+ *                        computed with an affine jump-ahead (no serial loop).
+ */
+int mcgpu_rng_real_step(mcgpu_context *ctx, int64_t n, uint64_t *seeds, double *reals,
+                        void *stream);
+int mcgpu_rng_seed_chain(mcgpu_context *ctx, uint64_t seed, int64_t n, uint64_t *out,
+                         void *stream);
+
+/* Evaluate the device log (which = 0) or exp (which = 1) used by the
+   transport kernels on n inputs (device pointers). */
+int mcgpu_math_eval(mcgpu_context *ctx, int which, int64_t n, const double *x, double *out,
+                    void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MCGPU_H */

@@ -1,0 +1,236 @@
+/*
+ * history.hip -- history-based transport kernel.
+ *
+ * One history per lane, run to termination with the loop condition of
+ * Layer::simulate_particle (src/layer.cpp:195-200) around the event of
+ * Layer::particle_step (src/layer.cpp:123-190).  Unlike the reference CUDA
+ * kernel (src/culayer_kernel.cu:19-113: one launch per 325 steps, host
+ * round trips and re-sorting between launches), the grid is persistent:
+ * a lane whose history ends takes the next particle index from a global
+ * work queue (one atomic per wave per refill), so lanes stay busy until the
+ * queue is empty and there is one launch per call.
+ *
+ * Tallies: each block accumulates the exact fixed-point tally of its cells
+ * in LDS (two 64-bit words per cell, 64-bit LDS atomics with carry) and adds
+ * it to the global 128-bit tally once at the end.  Integer sums are exact, so
+ * the result is independent of which lane, wave or block handled which
+ * history.  For layers too large for LDS the same atomics go to the global
+ * (L2-resident) tally.  The per-cell cross-section pairs are staged in LDS
+ * when they fit next to the tally, otherwise read through L2.
+ */
+#include "internal.h"
+
+namespace {
+
+__device__ __forceinline__ unsigned long long readlane64(unsigned long long v, int lane) {
+  unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, lane);
+  unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), lane);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+/* number of set bits of `mask` in lanes below this one */
+__device__ __forceinline__ int mbcnt64(unsigned long long mask) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__device__ __forceinline__ void load_state(const mc_transport_args &a, int64_t i, mc_state &p) {
+  if (a.aos) {
+    const mcgpu_particle *q = a.aos + i;
+    p.seed = q->seed;
+    p.x = q->x;
+    p.mu = q->mu;
+    p.wmc = q->wmc;
+    p.index = q->index;
+  } else {
+    p.seed = a.soa.seed[i];
+    p.x = a.soa.x[i];
+    p.mu = a.soa.mu[i];
+    p.wmc = a.soa.wmc[i];
+    p.index = a.soa.index[i];
+  }
+}
+
+__device__ __forceinline__ void store_state(const mc_transport_args &a, int64_t i,
+                                            const mc_state &p, int code) {
+  if (a.aos) {
+    mcgpu_particle *q = a.aos + i;
+    q->seed = p.seed;
+    q->x = p.x;
+    q->mu = p.mu;
+    q->wmc = p.wmc;
+    q->index = p.index;
+    if (a.result32) a.result32[i] = code;
+  } else {
+    a.soa.seed[i] = p.seed;
+    a.soa.x[i] = p.x;
+    a.soa.mu[i] = p.mu;
+    a.soa.wmc[i] = p.wmc;
+    a.soa.index[i] = p.index;
+    if (a.result8) a.result8[i] = (int8_t)code;
+  }
+}
+
+template <bool LDS_TALLY, bool LDS_CELLS>
+__global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refill_k) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int m = a.L.m;
+  unsigned long long *t_lo, *t_hi;
+  size_t off = 0;
+  if (LDS_TALLY) {
+    t_lo = reinterpret_cast<unsigned long long *>(smem);
+    t_hi = t_lo + m;
+    off = (size_t)16 * m;
+    for (int c = threadIdx.x; c < m; c += blockDim.x) {
+      t_lo[c] = 0;
+      t_hi[c] = 0;
+    }
+  } else {
+    t_lo = a.fx;
+    t_hi = a.fx + m;
+  }
+  const mc_cell *cells = a.cells;
+  if (LDS_CELLS) {
+    mc_cell *lc = reinterpret_cast<mc_cell *>(smem + off);
+    for (int c = threadIdx.x; c < m; c += blockDim.x) lc[c] = a.cells[c];
+    cells = lc;
+  }
+  __syncthreads();
+
+  const int lane = __lane_id();
+  mc_state p;
+  p.index = 0;
+  int64_t idx = 0;
+  bool has = false, done = false, capped = false;
+  uint64_t ev = 0, col = 0;
+  unsigned long long tot_ev = 0, tot_col = 0, nl = 0, nr = 0, nk = 0;
+
+  for (;;) {
+    if (!done) { /* wave-uniform */
+      const unsigned long long need = __ballot(!has);
+      if (need) {
+        const int cnt = __popcll(need);
+        const int leader = __ffsll((long long)need) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(&a.ctr->next, (unsigned long long)cnt);
+        base = readlane64(base, leader);
+        if (!has) {
+          const int64_t i = (int64_t)(base + (unsigned long long)mbcnt64(need));
+          if (i < a.n) {
+            load_state(a, i, p);
+            idx = i;
+            has = true;
+            ev = 0;
+            col = 0;
+          }
+        }
+        if (base + (unsigned long long)cnt >= (unsigned long long)a.n) done = true;
+      }
+    }
+    if (__ballot(has) == 0) break; /* wave-uniform */
+    if (has) {
+      for (int k = 0; k < refill_k; ++k) {
+        if (!mc_alive(p, a.L) || ev >= a.max_events) break;
+        const int c = p.index - a.L.index_start;
+        const mc_cell cell = cells[c];
+        bool coll;
+        double di;
+        const double dw = mc_event(p, cell, a.L.dx, coll, di);
+        ++ev;
+        col += coll ? 1u : 0u;
+        uint32_t f = 0;
+        const mc_fx v = mc_fx_from_double(dw, f);
+        if (f) {
+          atomicOr(&a.flags[c], f);
+          if (f & 8u) atomicOr(&a.ctr->status, 2u);
+        }
+        mc_fx_atomic_add(&t_lo[c], &t_hi[c], v);
+      }
+      const bool alive = mc_alive(p, a.L);
+      if (!alive || ev >= a.max_events) {
+        int code = mc_exit_code(p, a.L);
+        if (alive) { /* runaway history stopped by the guard */
+          capped = true;
+          code = 2;
+        }
+        store_state(a, idx, p, code);
+        if (a.events) a.events[idx] = ev;
+        if (a.collisions) a.collisions[idx] = col;
+        tot_ev += ev;
+        tot_col += col;
+        nl += code == -1;
+        nr += code == 1;
+        nk += code == 0;
+        has = false;
+      }
+    }
+  }
+
+  tot_ev = wave_sum(tot_ev);
+  tot_col = wave_sum(tot_col);
+  nl = wave_sum(nl);
+  nr = wave_sum(nr);
+  nk = wave_sum(nk);
+  if (lane == 0) {
+    atomicAdd(&a.ctr->events, tot_ev);
+    atomicAdd(&a.ctr->collisions, tot_col);
+    atomicAdd(&a.ctr->n_left, nl);
+    atomicAdd(&a.ctr->n_right, nr);
+    atomicAdd(&a.ctr->n_killed, nk);
+  }
+  if (capped) atomicOr(&a.ctr->status, 1u);
+
+  if (LDS_TALLY) {
+    __syncthreads();
+    for (int c = threadIdx.x; c < m; c += blockDim.x) {
+      mc_fx v;
+      v.lo = t_lo[c];
+      v.hi = t_hi[c];
+      mc_fx_atomic_add(&a.fx[c], &a.fx[m + c], v);
+    }
+  }
+}
+
+template <bool T, bool C>
+int launch_one(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_options &o,
+               hipStream_t s, size_t lds) {
+  auto kern = k_history<T, C>;
+  if (lds > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  int block = o.block_size > 0 ? o.block_size : (lds > 81920 ? 1024 : 256);
+  if (block > 1024 || block % 64) return mc_fail(ctx, MCGPU_ERR_ARG, "block_size must be a multiple of 64 and <= 1024");
+  int grid = o.grid_blocks;
+  if (grid <= 0) {
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, lds);
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    grid = per_cu * ctx->num_cus;
+    /* no more blocks than there are waves of work */
+    int64_t waves = (a.n + 63) / 64;
+    int64_t blocks_needed = (waves * 64 + block - 1) / block;
+    if (blocks_needed < grid) grid = (int)(blocks_needed > 0 ? blocks_needed : 1);
+  }
+  int k = o.events_per_refill > 0 ? o.events_per_refill : 8;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, s, a, k);
+  MC_HIP(ctx, hipGetLastError());
+  return MCGPU_OK;
+}
+
+} // namespace
+
+int mc_launch_history(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_options &o,
+                      hipStream_t s) {
+  const size_t m = (size_t)a.L.m;
+  const size_t tally = 16 * m, cells = sizeof(mc_cell) * m;
+  const bool lds_tally = tally <= ctx->lds_per_block;
+  const bool lds_cells = lds_tally && tally + cells <= 65536;
+  if (lds_tally && lds_cells) return launch_one<true, true>(ctx, a, o, s, tally + cells);
+  if (lds_tally) return launch_one<true, false>(ctx, a, o, s, tally);
+  return launch_one<false, false>(ctx, a, o, s, 0);
+}

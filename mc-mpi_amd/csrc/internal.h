@@ -1,0 +1,81 @@
+/*
+ * internal.h -- shared definitions of the library's translation units:
+ * the context, device counter block and kernel launchers.
+ */
+#ifndef MCGPU_INTERNAL_H
+#define MCGPU_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/mcgpu.h"
+#include "mc_device.h"
+
+/* Device-side counters of one transport call. */
+struct mc_counters {
+  unsigned long long next;       /* work-queue head (history kernel) */
+  unsigned long long events;
+  unsigned long long collisions;
+  unsigned long long n_left;
+  unsigned long long n_right;
+  unsigned long long n_killed;
+  unsigned int status;           /* bit 0: event cap hit, bit 1: tally range */
+  unsigned int pad_;
+  unsigned long long aux[8];     /* kernel-specific (event kernel pass data) */
+};
+
+struct mcgpu_context {
+  int device = 0;
+  int num_cus = 0;
+  size_t lds_per_block = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::string err;
+
+  /* grow-only scratch (device) */
+  void *cells = nullptr;      size_t cells_bytes = 0;     /* mc_cell[m] */
+  void *fx = nullptr;         size_t fx_bytes = 0;        /* u64[2m] tally */
+  void *flags = nullptr;      size_t flags_bytes = 0;     /* u32[m] */
+  void *counters = nullptr;                               /* mc_counters */
+  void *host_counters = nullptr;                          /* pinned mc_counters */
+  void *aos = nullptr;        size_t aos_bytes = 0;       /* staging mcgpu_particle[n] */
+  void *res32 = nullptr;      size_t res32_bytes = 0;     /* int32[n] */
+  void *sig_in = nullptr;     size_t sig_in_bytes = 0;    /* double[2m] sigs, ar */
+  void *w_in = nullptr;       size_t w_in_bytes = 0;      /* double[m] weights */
+  void *work = nullptr;       size_t work_bytes = 0;      /* event kernel banks */
+};
+
+/* grow-only device buffer */
+int mc_ensure(mcgpu_context *ctx, void **buf, size_t *cap, size_t need);
+int mc_fail(mcgpu_context *ctx, int status, const char *what, hipError_t e = hipSuccess);
+#define MC_HIP(ctx, call)                                          \
+  do {                                                             \
+    hipError_t e_ = (call);                                        \
+    if (e_ != hipSuccess) return mc_fail((ctx), MCGPU_ERR_HIP, #call, e_); \
+  } while (0)
+
+/* Arguments common to the transport kernels. */
+struct mc_transport_args {
+  mc_layer_k L;
+  int64_t n;
+  mcgpu_particle *aos; /* exactly one of aos / soa is used */
+  mcgpu_bank soa;
+  const mc_cell *cells;
+  unsigned long long *fx; /* [2m]: lo words then hi words */
+  uint32_t *flags;
+  int8_t *result8;
+  int32_t *result32;
+  uint64_t *events;
+  uint64_t *collisions;
+  mc_counters *ctr;
+  uint64_t max_events;
+};
+
+int mc_launch_history(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_options &o,
+                      hipStream_t s);
+int mc_launch_event(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_options &o,
+                    hipStream_t s);
+
+#endif

@@ -1,0 +1,214 @@
+/*
+ * mc_device.h -- device building blocks of the transport kernels:
+ * the reference's two LCGs, the per-cell cross-section table, the exact
+ * fixed-point tally arithmetic and one transport event.
+ *
+ * Everything here is FP64 and must be compiled with -ffp-contract=off so the
+ * arithmetic is exactly the sequence of IEEE operations the reference's
+ * expressions specify (no fused multiply-adds that the reference does not do).
+ */
+#ifndef MC_DEVICE_H
+#define MC_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mc_math.h"
+
+/* ------------------------------------------------------------------- RNG */
+/* src/random.cpp:8-16: s <- (G*s + C) mod 2^63; real = s * 2^-63 */
+#define MC_RNG_G 6364136223846793005ULL
+#define MC_RNG_C 1442695040888963407ULL
+/* src/random.cpp:20-29: seed chain */
+#define MC_RNGS_G 5177284530976225183ULL
+#define MC_RNGS_C 2096348467109453893ULL
+#define MC_MASK63 0x7FFFFFFFFFFFFFFFULL
+
+__device__ __forceinline__ double mc_rnd_real(uint64_t &seed) {
+  seed = (MC_RNG_G * seed + MC_RNG_C) & MC_MASK63;
+  return (double)seed * 0x1p-63; /* (double)seed rounds to nearest; scaling is exact */
+}
+
+__device__ __forceinline__ uint64_t mc_rnd_seed(uint64_t &seed) {
+  seed = (MC_RNGS_G * seed + MC_RNGS_C) & MC_MASK63;
+  return seed;
+}
+
+/* ------------------------------------------------------ cross sections */
+/* Per cell, the two products particle_step forms (src/layer.cpp:131-133):
+   sig_a = sigs*ar and sig_i = sigs*(1.0 - ar), computed once with the same
+   operations so every event reads the bit-identical values. */
+struct mc_cell {
+  double sig_a;
+  double sig_i;
+};
+
+__device__ __forceinline__ mc_cell mc_make_cell(double sigs, double ar) {
+  const double interaction_rate = 1.0 - ar;
+  mc_cell c;
+  c.sig_a = sigs * ar;
+  c.sig_i = sigs * interaction_rate;
+  return c;
+}
+
+/* ------------------------------------------------- exact fixed-point tally
+ * A term v maps to the signed 128-bit integer round_half_up(v * 2^88),
+ * stored as (lo, hi) two's complement.  Non-finite terms and |v| >= 2^39 set
+ * flag bits instead (1 NaN, 2 +inf, 4 -inf, 8 out of range).  Integer sums
+ * are exact, so the tally does not depend on the order in which lanes, waves
+ * and blocks add their terms. */
+#define MC_FX_FRAC 88
+
+struct mc_fx {
+  uint64_t lo;
+  uint64_t hi;
+};
+
+__device__ __forceinline__ mc_fx mc_fx_from_double(double v, uint32_t &flags) {
+  uint64_t b = mc_asu64(v);
+  uint32_t E = (uint32_t)(b >> 52) & 0x7FFu;
+  mc_fx r = {0, 0};
+  if (E == 0x7FFu) {
+    flags |= (b & 0x000FFFFFFFFFFFFFULL) ? 1u : ((b >> 63) ? 4u : 2u);
+    return r;
+  }
+  if (E >= 1023u + 39u) {
+    flags |= 8u;
+    return r;
+  }
+  if (E == 0u) return r; /* zero or subnormal: below 2^-89 -> 0 */
+  uint64_t M = (b & 0x000FFFFFFFFFFFFFULL) | 0x0010000000000000ULL;
+  int s = (int)E - 987; /* v = M * 2^(E-1075); v*2^88 = M * 2^(E-987) */
+  if (s >= 64) {
+    r.hi = M << (s - 64);
+  } else if (s > 0) {
+    r.lo = M << s;
+    r.hi = M >> (64 - s);
+  } else if (s == 0) {
+    r.lo = M;
+  } else if (s > -64) {
+    r.lo = ((M >> (-s - 1)) + 1) >> 1; /* round half up */
+  }
+  if (b >> 63) { /* negate (two's complement) */
+    r.lo = ~r.lo + 1;
+    r.hi = ~r.hi + (r.lo == 0 ? 1 : 0);
+  }
+  return r;
+}
+
+/* Correctly rounded conversion of the 128-bit sum back to double. */
+__host__ __device__ __forceinline__ double mc_fx_to_double(uint64_t lo, uint64_t hi) {
+  int neg = (int64_t)hi < 0;
+  if (neg) {
+    lo = ~lo + 1;
+    hi = ~hi + (lo == 0 ? 1 : 0);
+  }
+  double d;
+  if (hi == 0) {
+    d = (double)lo * 0x1p-88;
+  } else {
+    int lz = __builtin_clzll(hi);
+    uint64_t top = lz ? ((hi << lz) | (lo >> (64 - lz))) : hi;
+    uint64_t rest = lz ? (lo << lz) : lo;
+    top |= (rest != 0) ? 1u : 0u; /* sticky bit below the rounding position */
+    d = __builtin_ldexp((double)top, 64 - lz - MC_FX_FRAC);
+  }
+  return neg ? -d : d;
+}
+
+__host__ __device__ __forceinline__ double mc_fx_finalize(uint64_t lo, uint64_t hi, uint32_t flags) {
+  if ((flags & 1u) || (flags & 8u) || ((flags & 6u) == 6u)) return __builtin_nan("");
+  if (flags & 2u) return __builtin_inf();
+  if (flags & 4u) return -__builtin_inf();
+  return mc_fx_to_double(lo, hi);
+}
+
+/* 128-bit add into two 64-bit words with 64-bit atomics (LDS or global):
+   the carry out of the low word is detected from the returned old value. */
+__device__ __forceinline__ void mc_fx_atomic_add(unsigned long long *lo_word,
+                                                 unsigned long long *hi_word, mc_fx v) {
+  if ((v.lo | v.hi) == 0) return;
+  unsigned long long old = atomicAdd(lo_word, (unsigned long long)v.lo);
+  unsigned long long carry = (old + v.lo < old) ? 1ULL : 0ULL;
+  unsigned long long h = v.hi + carry;
+  if (h) atomicAdd(hi_word, h);
+}
+
+/* --------------------------------------------------------- particle state */
+struct mc_state {
+  uint64_t seed;
+  double x;
+  double mu;
+  double wmc;
+  int32_t index;
+};
+
+/* Layer constants read by the step (include/layer/layer.hpp:465-485) */
+struct mc_layer_k {
+  int32_t index_start;
+  int32_t m;
+  double dx;
+  double min_weight;
+};
+
+/* One transport event: src/layer.cpp:123-190 (Layer::particle_step) in FP64,
+   EPS_PRECISION = 1e-4, MAXREAL = DBL_MAX, log/exp correctly rounded.
+   Returns the absorbed weight dw and the local cell it is tallied to;
+   `collided` reports which branch was taken. */
+#define MC_EPS 1e-4
+#define MC_MAXREAL 1.7976931348623157e308
+
+__device__ __forceinline__ double mc_event(mc_state &p, const mc_cell cell, const double dx,
+                                           bool &collided, double &di_out) {
+  const double h = mc_rnd_real(p.seed);
+  double di = cell.sig_i > MC_EPS ? -mc_log(h) / cell.sig_i : MC_MAXREAL;
+
+  int index_new;
+  double x_new_edge;
+  if (p.mu < 0) {
+    index_new = p.index - 1;
+    x_new_edge = p.index * dx;
+  } else {
+    index_new = p.index + 1;
+    x_new_edge = (p.index + 1) * dx;
+  }
+
+  double di_edge = MC_MAXREAL;
+  if (p.mu < -MC_EPS || MC_EPS < p.mu) di_edge = (x_new_edge - p.x) / p.mu;
+
+  collided = di < di_edge;
+  if (collided) {
+    index_new = p.index;
+    p.x += di * p.mu;
+    p.mu = 2 * mc_rnd_real(p.seed) - 1;
+  } else {
+    di = di_edge;
+    p.x = x_new_edge;
+  }
+
+  const double dw = (1 - mc_exp(-cell.sig_a * di)) * p.wmc;
+  p.wmc -= dw;
+  p.index = index_new;
+  di_out = di;
+  return dw;
+}
+
+/* src/layer.cpp:195-197: the history continues while this holds */
+__device__ __forceinline__ bool mc_alive(const mc_state &p, const mc_layer_k &L) {
+  return (p.wmc >= L.min_weight) && (p.index < L.index_start + L.m) && (p.index >= L.index_start);
+}
+
+/* src/layer.cpp:202-217: exit code of a finished history */
+__device__ __forceinline__ int mc_exit_code(const mc_state &p, const mc_layer_k &L) {
+  if (p.index == L.index_start - 1) return -1;
+  if (p.index == L.index_start + L.m) return +1;
+  if (p.wmc < L.min_weight) return 0;
+  return -1;
+}
+
+/* -------------------------------------------------------------- wave utils */
+__device__ __forceinline__ int mc_lane() { return __lane_id(); }
+
+__device__ __forceinline__ unsigned long long mc_ballot(bool pred) { return __ballot(pred); }
+
+#endif /* MC_DEVICE_H */

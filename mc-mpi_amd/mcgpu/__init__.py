@@ -1,0 +1,168 @@
+"""mcgpu: Python access to the MI355X particle-transport library (libmcgpu.so).
+
+Thin ctypes wrapper over the C ABI of include/mcgpu.h; every call goes to the
+HIP kernels.  Device memory for the device-resident entry points is plain
+torch tensors (their data pointers are passed through the C ABI).
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (ERR_ARG, ERR_EVENT_CAP, ERR_HIP, ERR_NO_DEVICE, ERR_NOMEM, ERR_RANGE,
+                   KERNEL_EVENT, KERNEL_HISTORY, OK, PARTICLE, Bank, McgpuError, Options, Stats,
+                   lib, status_string)
+
+__all__ = ["Context", "PARTICLE", "McgpuError", "KERNEL_HISTORY", "KERNEL_EVENT", "lib",
+           "bank_to_device", "bank_from_device", "make_options"]
+
+_KERNELS = {"history": KERNEL_HISTORY, "event": KERNEL_EVENT}
+
+
+def make_options(kernel="history", block_size=0, grid_blocks=0, events_per_refill=0,
+                 compaction_interval=0, max_events_per_history=0):
+    o = Options()
+    lib().mcgpu_default_options(C.byref(o))
+    o.kernel = _KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
+    o.block_size = block_size
+    o.grid_blocks = grid_blocks
+    o.events_per_refill = events_per_refill
+    o.compaction_interval = compaction_interval
+    o.max_events_per_history = max_events_per_history
+    return o
+
+
+def _stream(stream):
+    """The stream a call is ordered on: the given one, else torch's current
+    stream (so the call is ordered after the torch ops that filled its
+    inputs); 0 / None is HIP's default stream."""
+    if stream is not None:
+        return stream
+    try:
+        import torch
+        if torch.cuda.is_initialized():
+            return torch.cuda.current_stream().cuda_stream
+    except ImportError:
+        pass
+    return None
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if isinstance(t, np.ndarray):
+        return t.ctypes.data
+    return t.data_ptr()
+
+
+def bank_to_device(particles, device="cuda"):
+    """numpy PARTICLE array -> dict of torch tensors (SoA bank) on `device`."""
+    import torch
+    p = np.ascontiguousarray(particles, dtype=PARTICLE)
+    return {
+        "seed": torch.from_numpy(p["seed"].view(np.int64).copy()).to(device),
+        "x": torch.from_numpy(p["x"].copy()).to(device),
+        "mu": torch.from_numpy(p["mu"].copy()).to(device),
+        "wmc": torch.from_numpy(p["wmc"].copy()).to(device),
+        "index": torch.from_numpy(p["index"].copy()).to(device),
+    }
+
+
+def bank_from_device(bank):
+    """dict of torch tensors -> numpy PARTICLE array."""
+    n = bank["x"].numel()
+    out = np.zeros(n, dtype=PARTICLE)
+    out["seed"] = bank["seed"].cpu().numpy().view(np.uint64)
+    out["x"] = bank["x"].cpu().numpy()
+    out["mu"] = bank["mu"].cpu().numpy()
+    out["wmc"] = bank["wmc"].cpu().numpy()
+    out["index"] = bank["index"].cpu().numpy()
+    return out
+
+
+class Context:
+    """A device context (stream + reusable device scratch) of libmcgpu."""
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        rc = lib().mcgpu_context_create(device, C.byref(self._h))
+        if rc != OK:
+            raise McgpuError(rc, status_string(rc))
+        self.device = device
+
+    def close(self):
+        if self._h:
+            lib().mcgpu_context_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def error(self):
+        return lib().mcgpu_context_error(self._h).decode()
+
+    def check(self, rc, allow=()):
+        if rc != OK and rc not in allow:
+            raise McgpuError(rc, f"{status_string(rc)}: {self.error()}")
+        return rc
+
+    # ------------------------------------------------------------ transport
+    def simulate(self, particles, sigs, absorption_rates, weights_absorbed, min_index, max_index,
+                 dx, particle_min_weight, options=None, allow=()):
+        """Host-buffer drop-in (mcgpu_simulate).  `particles` (PARTICLE array)
+        and `weights_absorbed` (float64) are updated in place.  Returns
+        (result int32 array indexed by position, stats dict, status)."""
+        assert particles.dtype == PARTICLE and particles.flags.c_contiguous
+        assert weights_absorbed.dtype == np.float64 and weights_absorbed.flags.c_contiguous
+        sigs = np.ascontiguousarray(sigs, np.float64)
+        ar = np.ascontiguousarray(absorption_rates, np.float64)
+        n = len(particles)
+        result = np.zeros(n, np.int32)
+        st = Stats()
+        opt = options if options is not None else make_options()
+        rc = lib().mcgpu_simulate(self._h, n, particles.ctypes.data, sigs.ctypes.data,
+                                  ar.ctypes.data, weights_absorbed.ctypes.data, min_index,
+                                  max_index, dx, particle_min_weight, result.ctypes.data,
+                                  C.byref(opt), C.byref(st))
+        self.check(rc, allow)
+        return result, st.as_dict(), rc
+
+    def simulate_device(self, bank, sigs, absorption_rates, weights_absorbed, min_index,
+                        max_index, dx, particle_min_weight, result=None, events=None,
+                        collisions=None, options=None, stream=None, allow=()):
+        """Device-resident transport (mcgpu_simulate_device) on a dict-of-tensors
+        bank; tensors are updated in place.  Returns (stats dict, status)."""
+        b = Bank(_ptr(bank["seed"]), _ptr(bank["x"]), _ptr(bank["mu"]), _ptr(bank["wmc"]),
+                 _ptr(bank["index"]))
+        n = bank["x"].numel()
+        st = Stats()
+        opt = options if options is not None else make_options()
+        rc = lib().mcgpu_simulate_device(self._h, n, C.byref(b), _ptr(sigs), _ptr(absorption_rates),
+                                         _ptr(weights_absorbed), min_index, max_index, dx,
+                                         particle_min_weight, _ptr(result), _ptr(events),
+                                         _ptr(collisions), C.byref(opt), C.byref(st),
+                                         _stream(stream))
+        self.check(rc, allow)
+        return st.as_dict(), rc
+
+    # ------------------------------------------------------------ helpers
+    def rng_real_step(self, seeds, reals, stream=None):
+        self.check(lib().mcgpu_rng_real_step(self._h, seeds.numel(), _ptr(seeds), _ptr(reals),
+                                             _stream(stream)))
+
+    def rng_seed_chain(self, seed, out, stream=None):
+        self.check(lib().mcgpu_rng_seed_chain(self._h, seed, out.numel(), _ptr(out),
+                                              _stream(stream)))
+
+    def math_eval(self, which, x, out, stream=None):
+        w = {"log": 0, "exp": 1}[which]
+        self.check(lib().mcgpu_math_eval(self._h, w, x.numel(), _ptr(x), _ptr(out),
+                                         _stream(stream)))

@@ -1,0 +1,103 @@
+"""ctypes declarations of the C ABI in include/mcgpu.h (libmcgpu.so).
+
+The library is loaded from the package directory (mc-mpi_amd/libmcgpu.so).
+If it is missing the import fails with an explicit error: there is no
+fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "libmcgpu.so")
+
+OK = 0
+ERR_ARG = -1
+ERR_HIP = -2
+ERR_NOMEM = -3
+ERR_NO_DEVICE = -4
+ERR_EVENT_CAP = -5
+ERR_RANGE = -6
+
+KERNEL_HISTORY = 0
+KERNEL_EVENT = 1
+
+# include/mcgpu.h mcgpu_particle (include/types/particle.hpp:7-17, FP64): 40 B
+PARTICLE = np.dtype([("seed", "<u8"), ("x", "<f8"), ("mu", "<f8"), ("wmc", "<f8"),
+                     ("index", "<i4"), ("pad_", "<i4")])
+assert PARTICLE.itemsize == 40
+
+
+class Bank(C.Structure):
+    _fields_ = [("seed", C.c_void_p), ("x", C.c_void_p), ("mu", C.c_void_p), ("wmc", C.c_void_p),
+                ("index", C.c_void_p)]
+
+
+class Options(C.Structure):
+    _fields_ = [("kernel", C.c_int32), ("block_size", C.c_int32), ("grid_blocks", C.c_int32),
+                ("events_per_refill", C.c_int32), ("compaction_interval", C.c_int32),
+                ("reserved_", C.c_int32), ("max_events_per_history", C.c_uint64)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("events", C.c_uint64), ("collisions", C.c_uint64), ("n_left", C.c_int64),
+                ("n_right", C.c_int64), ("n_killed", C.c_int64), ("kernel_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class McgpuError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"mcgpu status {status}: {message}")
+        self.status = status
+
+
+_lib = None
+
+# name -> (restype, argtypes); every symbol include/mcgpu.h declares
+vp, i32, i64, u64, dbl, cp = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64, C.c_double, C.c_char_p
+SIGNATURES = {
+    "mcgpu_context_create": (i32, [C.c_int, C.POINTER(vp)]),
+    "mcgpu_context_destroy": (i32, [vp]),
+    "mcgpu_status_string": (cp, [C.c_int]),
+    "mcgpu_context_error": (cp, [vp]),
+    "mcgpu_default_options": (None, [C.POINTER(Options)]),
+    "mcgpu_version": (cp, []),
+    "mcgpu_simulate": (i32, [vp, i32, vp, vp, vp, vp, i32, i32, dbl, dbl, vp,
+                             C.POINTER(Options), C.POINTER(Stats)]),
+    "mcgpu_simulate_device": (i32, [vp, i64, C.POINTER(Bank), vp, vp, vp, i32, i32, dbl, dbl,
+                                    vp, vp, vp, C.POINTER(Options), C.POINTER(Stats), vp]),
+    "mcgpu_rng_real_step": (i32, [vp, i64, vp, vp, vp]),
+    "mcgpu_rng_seed_chain": (i32, [vp, u64, i64, vp, vp]),
+    "mcgpu_math_eval": (i32, [vp, C.c_int, i64, vp, vp, vp]),
+}
+
+
+def lib():
+    """Load libmcgpu.so; raises ImportError when the HIP library is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} not found: build the HIP library first "
+                "(python -c 'import __graft_entry__ as g; g.build()' or make -C mc-mpi_amd)")
+        # libmcgpu.so and torch both need libamdhip64.so.7; the process can
+        # hold only one library of that name, and torch initialises its GPUs
+        # only with the runtime it ships.  Load torch's first so both share it.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def status_string(status):
+    return lib().mcgpu_status_string(status).decode()

@@ -36,46 +36,80 @@ def build(quiet=True):
         print(out.stdout)
 
 
+def _open(path):
+    L = C.CDLL(path)
+    vp, i64, i32, dbl, flt = C.c_void_p, C.c_int64, C.c_int, C.c_double, C.c_float
+    for pfx, real in (("orc32_", flt), ("orc64_", dbl)):
+        getattr(L, pfx + "layer_new").restype = vp
+        getattr(L, pfx + "layer_new").argtypes = [real, real, i32, i32, real]
+        getattr(L, pfx + "decompose_domain").restype = vp
+        getattr(L, pfx + "decompose_domain").argtypes = [real, real, real, i32, i32, i32, i32, real]
+        getattr(L, pfx + "layer_free").argtypes = [vp]
+        getattr(L, pfx + "create_particles").argtypes = [vp, real, real, i32, C.c_uint64]
+        getattr(L, pfx + "simulate").argtypes = [vp, i32, i32]
+        getattr(L, pfx + "dump_WA").argtypes = [vp, C.c_char_p]
+        getattr(L, pfx + "dump_WA").restype = i32
+        getattr(L, pfx + "layer_array").argtypes = [vp, i32, C.POINTER(vp)]
+        getattr(L, pfx + "layer_array").restype = i64
+        getattr(L, pfx + "layer_int").argtypes = [vp, i32]
+        getattr(L, pfx + "layer_int").restype = i64
+        getattr(L, pfx + "layer_real").argtypes = [vp, i32]
+        getattr(L, pfx + "layer_real").restype = dbl
+        getattr(L, pfx + "layer_push").argtypes = [vp, vp, i64]
+        getattr(L, pfx + "layer_push").restype = i32
+        getattr(L, pfx + "simulate_bank").restype = i32
+    L.orc64_layer_set_tally_exact.argtypes = [vp, i32]
+    L.orc64_simulate_bank_fx.argtypes = [vp, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp, i32]
+    L.orc64_simulate_bank_fx.restype = i32
+    L.orc32_simulate_bank.argtypes = [vp, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp, i32]
+    L.orc_set_math_mode.argtypes = [i32]
+    L.orc_rnd_seed_chain.argtypes = [C.c_uint64, i64, vp]
+    L.orc32_rnd_real_step.argtypes = [vp, i64, vp]
+    L.orc64_rnd_real_step.argtypes = [vp, i64, vp]
+    L.orc64_math_eval.argtypes = [i32, vp, i64, vp]
+    L.orc_fx_convert.argtypes = [vp, i64, vp, vp]
+    L.orc_fx_finalize.argtypes = [vp, vp, i64, vp]
+    L.orc_math_unresolved.restype = C.c_ulonglong
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        vp, i64, i32, dbl, flt = C.c_void_p, C.c_int64, C.c_int, C.c_double, C.c_float
-        for pfx, real in (("orc32_", flt), ("orc64_", dbl)):
-            getattr(L, pfx + "layer_new").restype = vp
-            getattr(L, pfx + "layer_new").argtypes = [real, real, i32, i32, real]
-            getattr(L, pfx + "decompose_domain").restype = vp
-            getattr(L, pfx + "decompose_domain").argtypes = [real, real, real, i32, i32, i32, i32, real]
-            getattr(L, pfx + "layer_free").argtypes = [vp]
-            getattr(L, pfx + "create_particles").argtypes = [vp, real, real, i32, C.c_uint64]
-            getattr(L, pfx + "simulate").argtypes = [vp, i32, i32]
-            getattr(L, pfx + "dump_WA").argtypes = [vp, C.c_char_p]
-            getattr(L, pfx + "dump_WA").restype = i32
-            getattr(L, pfx + "layer_array").argtypes = [vp, i32, C.POINTER(vp)]
-            getattr(L, pfx + "layer_array").restype = i64
-            getattr(L, pfx + "layer_int").argtypes = [vp, i32]
-            getattr(L, pfx + "layer_int").restype = i64
-            getattr(L, pfx + "layer_real").argtypes = [vp, i32]
-            getattr(L, pfx + "layer_real").restype = dbl
-            getattr(L, pfx + "layer_push").argtypes = [vp, vp, i64]
-            getattr(L, pfx + "layer_push").restype = i32
-            getattr(L, pfx + "simulate_bank").restype = i32
-        L.orc64_layer_set_tally_exact.argtypes = [vp, i32]
-        L.orc64_simulate_bank_fx.argtypes = [vp, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp, i32]
-        L.orc64_simulate_bank_fx.restype = i32
-        L.orc32_simulate_bank.argtypes = [vp, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp, i32]
-        L.orc_set_math_mode.argtypes = [i32]
-        L.orc_rnd_seed_chain.argtypes = [C.c_uint64, i64, vp]
-        L.orc32_rnd_real_step.argtypes = [vp, i64, vp]
-        L.orc64_rnd_real_step.argtypes = [vp, i64, vp]
-        L.orc64_math_eval.argtypes = [i32, vp, i64, vp]
-        L.orc_fx_convert.argtypes = [vp, i64, vp, vp]
-        L.orc_fx_finalize.argtypes = [vp, vp, i64, vp]
-        L.orc_math_unresolved.restype = C.c_ulonglong
-        _lib = L
+        _lib = _open(LIB_PATH)
     return _lib
+
+
+def build_mutant(k, outdir):
+    """Compile the oracle with planted mistake number k (ORC_MUTANT in
+    orc_impl.inc) into outdir and return its ctypes handle.  Tests only."""
+    path = os.path.join(outdir, f"liboracle_mutant{k}.so")
+    cmd = ["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
+           f"-DORC_MUTANT={k}", "-o", path, os.path.join(HERE, "oracle.c"),
+           os.path.join(HERE, "orc_math.c"), "-lquadmath", "-lm"]
+    out = subprocess.run(cmd, capture_output=True, text=True)
+    if out.returncode != 0:
+        raise RuntimeError(out.stderr)
+    return _open(path)
+
+
+class using:
+    """Context manager: route this module's calls to another oracle build."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    def __enter__(self):
+        global _lib
+        self.saved = lib()
+        _lib = self.handle
+        return self.handle
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
 
 
 def ref_available():

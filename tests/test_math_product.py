@@ -1,0 +1,92 @@
+"""The product's correctly rounded log/exp (mc-mpi_amd/csrc/mc_math.h),
+compiled for the host with gcc, against the oracle's independent correctly
+rounded evaluation (x87 extended + binary128).  The same source is compiled
+into the HIP kernels; tests/test_parity_gpu.py repeats the check on the
+device."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+SHIM = r"""
+#include <stdint.h>
+#include "mc_math.h"
+void pm_log(const double *x, long n, double *y) { for (long i = 0; i < n; ++i) y[i] = mc_log(x[i]); }
+void pm_exp(const double *x, long n, double *y) { for (long i = 0; i < n; ++i) y[i] = mc_exp(x[i]); }
+"""
+
+
+@pytest.fixture(scope="module")
+def pm(tmp_path_factory):
+    d = tmp_path_factory.mktemp("pm")
+    src = d / "pm.c"
+    src.write_text(SHIM)
+    so = d / "libpm.so"
+    subprocess.run(["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-ffp-contract=off",
+                    "-I", os.path.join(ROOT, "mc-mpi_amd", "csrc"), "-o", str(so), str(src), "-lm"],
+                   check=True)
+    L = C.CDLL(str(so))
+    for f in (L.pm_log, L.pm_exp):
+        f.argtypes = [C.c_void_p, C.c_long, C.c_void_p]
+
+    def run(which, x):
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.empty_like(x)
+        getattr(L, "pm_" + which)(x.ctypes.data, len(x), y.ctypes.data)
+        return y
+    return run
+
+
+def _same(a, b):
+    return (a.view(np.uint64) == b.view(np.uint64)) | (np.isnan(a) & np.isnan(b))
+
+
+def test_log_random_rnd_real_outputs(pm, oracle):
+    """log is applied to rnd_real outputs h = (double)s * 2^-63."""
+    rng = np.random.default_rng(1)
+    s = rng.integers(0, 2 ** 63, 400_000, dtype=np.uint64)
+    s >>= rng.integers(0, 60, 400_000).astype(np.uint64)
+    h = s.astype(np.float64) * 2.0 ** -63
+    h = h[h > 0]
+    assert np.all(_same(pm("log", h), oracle.math_eval("log", h)))
+
+
+def test_log_near_one_and_structured(pm, oracle):
+    k = np.arange(1, 100_001, dtype=np.float64)
+    x = np.concatenate([1.0 + k * 2.0 ** -52, 1.0 - k * 2.0 ** -53,
+                        1.0 + (np.random.default_rng(2).random(100_000) - 0.5) * 2.0 ** -8])
+    assert np.all(_same(pm("log", x), oracle.math_eval("log", x)))
+
+
+def test_log_full_range_and_specials(pm, oracle):
+    rng = np.random.default_rng(3)
+    bits = rng.integers(0, 2 ** 63, 200_000, dtype=np.uint64)
+    x = bits.view(np.float64)
+    x = np.concatenate([x, [0.0, -0.0, 1.0, np.inf, -1.0, np.nan, 5e-324, 2.2250738585072014e-308,
+                            1.7976931348623157e308]])
+    assert np.all(_same(pm("log", x), oracle.math_eval("log", x)))
+
+
+def test_exp_transport_range(pm, oracle):
+    """exp is applied to -sig_a * di <= 0 of every magnitude."""
+    rng = np.random.default_rng(4)
+    x = -rng.random(400_000) * 10.0 ** rng.uniform(-18, 2.9, 400_000)
+    assert np.all(_same(pm("exp", x), oracle.math_eval("exp", x)))
+
+
+def test_exp_structured_tiny_and_subnormal(pm, oracle):
+    k = np.arange(1, 20_001, dtype=np.float64)
+    x = np.concatenate([k * 2.0 ** -53, -k * 2.0 ** -53, k * 2.0 ** -60, -k * 2.0 ** -40,
+                        -708.0 - np.random.default_rng(5).random(50_000) * 38.0,
+                        [0.0, -0.0, np.inf, -np.inf, np.nan, -745.1332191019411, -745.14,
+                         709.782712893384, 709.79, 2.0 ** -54, -(2.0 ** -54)]])
+    assert np.all(_same(pm("exp", x), oracle.math_eval("exp", x)))
+
+
+def test_exp_full_range(pm, oracle):
+    x = (np.random.default_rng(6).random(300_000) - 0.5) * 1500.0
+    assert np.all(_same(pm("exp", x), oracle.math_eval("exp", x)))

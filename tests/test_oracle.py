@@ -1,0 +1,389 @@
+"""Pins of the CPU oracle (no GPU).
+
+The oracle restates src/layer.cpp and src/random.cpp.  These tests check it
+against everything that fixes its answers independently of its own code:
+  - the reference's golden file data/test_layer_target_WA.out (FP32, byte
+    exact), copied to tests/golden/ (src/test_layer.cpp:41-68 recipe);
+  - the reference itself, compiled from its sources into oracle/_ref (FP32,
+    per history, bit exact) -- skipped where that build is absent;
+  - big-integer arithmetic for the RNG, exact rational arithmetic for the
+    fixed-point tally, 40-digit decimal arithmetic for log/exp;
+  - closed forms: pure absorber, pure scatterer, streaming, kill rule,
+    order invariance, and the statistical profile tex/data/WA_1000_1000000.out.
+"""
+import decimal
+import os
+import random
+import tempfile
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+X_INI_F32 = np.float32(np.sqrt(np.float32(2.0)) / 2.0)  # src/test_layer.cpp:45
+
+
+# ------------------------------------------------------------------ golden
+
+@pytest.mark.parametrize("nthread", [1, 4])
+def test_fp32_oracle_reproduces_reference_golden_file(oracle, nthread):
+    """src/test_layer.cpp:41-68: decompose_domain(0,1,sqrt(2)/2,1,0,100,100,0),
+    simulate(-1), dump_WA, byte compare."""
+    L = oracle.OracleLayer.decompose_domain(32, 0.0, 1.0, X_INI_F32, 1, 0, 100, 100, 0.0)
+    L.simulate(-1, nthread)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "WA.out")
+        assert L.dump_WA(p) == 0
+        got = open(p, "rb").read()
+    want = open(os.path.join(GOLDEN, "test_layer_target_WA.out"), "rb").read()
+    assert got == want
+    assert L.nb_disabled == 100 and L.nb_active == 0
+
+
+# ------------------------------------------------- against the reference build
+
+needs_ref = pytest.mark.skipif(
+    not os.path.exists(os.path.join(os.path.dirname(__file__), "..", "oracle", "_ref", "libref.so")),
+    reason="oracle/_ref not built (reference sources absent)")
+
+
+@needs_ref
+def test_reference_build_reproduces_its_golden_file(oracle):
+    R = oracle.RefLayer.decompose_domain(0.0, 1.0, X_INI_F32, 1, 0, 100, 100, 0.0)
+    R.simulate(-1, 1)
+    with tempfile.TemporaryDirectory() as d:
+        got = open(R.dump_WA_in(d), "rb").read()
+    assert got == open(os.path.join(GOLDEN, "test_layer_target_WA.out"), "rb").read()
+
+
+def _fp32_bank(n, x, index, mu_fn, w, seed0=5127801):
+    b = np.zeros(n, dtype=[("seed", "<u8"), ("x", "<f4"), ("mu", "<f4"), ("wmc", "<f4"),
+                           ("index", "<i4")])
+    s = seed0
+    for i in range(n):
+        s = (5177284530976225183 * s + 2096348467109453893) % (1 << 63)
+        b["seed"][i] = s
+    b["x"] = x
+    b["index"] = index
+    b["mu"] = mu_fn(n)
+    b["wmc"] = w
+    return b
+
+
+REF_CASES = ["default_law_point", "slab_face_varied_ar", "interior_layer_varied_ar"]
+
+
+def _ref_case(oracle, case):
+    """(reference Layer, bank, index_start, m, min_w) of one comparison case;
+    the varied cases overwrite the public sigs / absorption_rates vectors
+    (include/layer/layer.hpp:101-104) so that sig_a != sig_i."""
+    rng = np.random.default_rng(7)
+    if case == "default_law_point":
+        R = oracle.RefLayer.new(0.0, 1.0, 0, 100, 1e-3)
+        start, m, min_w = 0, 100, 1e-3
+        bank = _fp32_bank(3000, np.float32(0.3), 30, lambda n: rng.uniform(-1, 1, n), 1.0)
+    elif case == "slab_face_varied_ar":
+        R = oracle.RefLayer.new(0.0, 1.0, 0, 50, 1e-2)
+        start, m, min_w = 0, 50, 1e-2
+        bank = _fp32_bank(3000, 0.0, 0, lambda n: np.sqrt(rng.uniform(0, 1, n)), 1.0)
+    else:
+        # rank 1 of 3 of a 300-cell slab: cells [100, 200)
+        R = oracle.RefLayer.decompose_domain(0.0, 1.0, np.float32(0.9), 3, 1, 300, 10, 1e-2)
+        start, m, min_w = 100, 100, 1e-2
+        bank = _fp32_bank(2000, np.float32(1.0 / 3.0), 150, lambda n: rng.uniform(-1, 1, n), 0.5)
+    if case != "default_law_point":
+        R.sigs[:] = rng.lognormal(0.5, 0.5, m).astype(np.float32)
+        R.absorption_rates[:] = rng.uniform(0.05, 0.95, m).astype(np.float32)
+    return R, bank, start, m, min_w
+
+
+def _ref_per_history_equal(oracle, case):
+    R, bank, start, m, min_w = _ref_case(oracle, case)
+    sigs, ar, dx = R.sigs.copy(), R.absorption_rates.copy(), np.float32(R.dx)
+    ref_bank, ref_res, ref_ev, ref_tally = R.transport_bank(bank, start, m)
+    o = oracle.simulate_bank(32, start, m, dx, min_w, sigs, ar, bank, nthread=1)
+    return (np.array_equal(o["bank"].view(np.uint8), ref_bank.view(np.uint8))
+            and np.array_equal(o["result"], ref_res) and np.array_equal(o["events"], ref_ev)
+            and np.array_equal(o["weights"].view(np.uint32), ref_tally.view(np.uint32)))
+
+
+@needs_ref
+@pytest.mark.parametrize("case", REF_CASES)
+def test_fp32_oracle_matches_reference_per_history(oracle, case):
+    """Every history's final Particle, exit code and event count, and the
+    tallies, bit-identical between the FP32 oracle and the reference's own
+    Layer::particle_step / simulate_particle."""
+    assert _ref_per_history_equal(oracle, case)
+
+
+@needs_ref
+def test_fp32_layer_object_matches_reference(oracle):
+    """Layer-level semantics: lazy creation in MAX_PARTICLES_VECTOR chunks,
+    partial simulate(n) calls, bucketing order of particles_left/right,
+    nb_disabled, interior layer (escape buffers kept)."""
+    args = (np.float32(0.0), np.float32(1.0), np.float32(0.55), 3, 1, 90, 500, np.float32(1e-3))
+    R = oracle.RefLayer.decompose_domain(*args)
+    O = oracle.OracleLayer.decompose_domain(32, *args)
+    for n in (37, 100, 1, 250, -1):
+        R.simulate(n, 1)
+        O.simulate(n, 1)
+        assert O.nb_active == R.nb_active
+        assert O.nb_disabled == R.nb_disabled
+        for name in ("particles", "particles_left", "particles_right"):
+            a, b = getattr(O, name), getattr(R, name)
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), name
+        assert np.array_equal(O.weights_absorbed.view(np.uint32), R.weights_absorbed.view(np.uint32))
+    assert len(O.particles_left) + len(O.particles_right) > 0
+
+
+# ------------------------------------------------------------------ RNG pins
+
+def test_rng_seed_chain_against_big_integers(oracle):
+    """src/test_curandom.cu:23-31 recipe (seed 30061994) in exact integers."""
+    got = oracle.rnd_seed_chain(30061994, 2000)
+    s = 30061994
+    for i in range(2000):
+        s = (5177284530976225183 * s + 2096348467109453893) % (1 << 63)
+        assert int(got[i]) == s
+
+
+def test_rng_real_step_against_exact_arithmetic(oracle):
+    seeds = oracle.rnd_seed_chain(30061994, 5000)
+    s_after, r64 = oracle.rnd_real_step(seeds, 64)
+    _, r32 = oracle.rnd_real_step(seeds, 32)
+    for i in range(5000):
+        s = (6364136223846793005 * int(seeds[i]) + 1442695040888963407) % (1 << 63)
+        assert int(s_after[i]) == s
+        # (real_t)s rounds to nearest; * 2^-63 is exact
+        assert Fraction(float(r64[i])) == Fraction(float(s)) / (1 << 63)
+        assert float(r32[i]) == float(np.float32(s)) * 2.0 ** -63
+
+
+def test_jump_ahead_seed_chain_matches_serial(oracle):
+    from mcgpu import configs
+    assert np.array_equal(configs.seed_chain(3000), oracle.rnd_seed_chain(5127801, 3000))
+    assert np.array_equal(configs.seed_chain(100, start=2900),
+                          oracle.rnd_seed_chain(5127801, 3000)[2900:])
+
+
+@needs_ref
+def test_rng_matches_reference_rnd_real(oracle):
+    import ctypes as C
+    R = oracle.ref()
+    seeds = oracle.rnd_seed_chain(30061994, 2000)
+    _, r32 = oracle.rnd_real_step(seeds, 32)
+    for i in range(2000):
+        s = C.c_ulonglong(int(seeds[i]))
+        assert R.ref_rnd_real(C.byref(s)) == r32[i]
+
+
+# ------------------------------------------------------------- math pins
+
+def _cr_double(d):
+    """Correctly rounded double of a Decimal (Python's float() of a decimal
+    string is correctly rounded)."""
+    return float(d)
+
+
+def test_oracle_log_exp_are_correctly_rounded(oracle):
+    decimal.getcontext().prec = 60
+    rng = random.Random(11)
+    xs = [rng.random() * 2.0 ** -rng.randint(0, 63) for _ in range(1500)]
+    xs += [1.0 + (rng.random() - 0.5) * 2.0 ** -rng.randint(1, 52) for _ in range(500)]
+    got = oracle.math_eval("log", np.array(xs))
+    for x, g in zip(xs, got):
+        assert g == _cr_double(decimal.Decimal(x).ln()), x
+    ys = [-rng.random() * 10.0 ** rng.uniform(-12, 2.8) for _ in range(2000)]
+    ys += [(rng.random() - 0.5) * 2.0 ** -rng.randint(20, 60) for _ in range(300)]
+    ys += [2.0 ** -53, -(2.0 ** -53), 2.0 ** -54]
+    got = oracle.math_eval("exp", np.array(ys))
+    for y, g in zip(ys, got):
+        assert g == _cr_double(decimal.Decimal(y).exp()), y
+    assert oracle.math_unresolved() == 0
+
+
+def test_glibc_mode_differs_rarely_from_correct_rounding(oracle):
+    """Measures the reference's libm against correct rounding (glibc is not
+    correctly rounded); the rate is small but nonzero."""
+    rng = np.random.default_rng(3)
+    h = rng.random(200_000)
+    cr = oracle.math_eval("log", h)
+    gl = oracle.math_eval("log", h, math=oracle.MATH_GLIBC)
+    frac = np.mean(cr != gl)
+    assert frac < 5e-3
+    x = -rng.random(200_000) * 5
+    frac_e = np.mean(oracle.math_eval("exp", x) != oracle.math_eval("exp", x, math=oracle.MATH_GLIBC))
+    assert frac_e < 5e-3
+
+
+# ------------------------------------------------------- fixed-point tally
+
+def test_fixed_point_conversion_against_rationals(oracle):
+    rng = np.random.default_rng(5)
+    vals = np.concatenate([rng.random(3000) * 10.0 ** rng.uniform(-30, 11, 3000),
+                           -rng.random(300), [0.0, -0.0, 2.0 ** -89, 2.0 ** -90, 3 * 2.0 ** -90,
+                                              2.0 ** 38, 5e-324]])
+    fx, fl = oracle.fx_convert(vals)
+    for i, v in enumerate(vals):
+        q = Fraction(float(v)) * (1 << 88)
+        # round half up on the magnitude, sign applied after
+        mag = abs(q)
+        r = int(mag) + (1 if mag - int(mag) >= Fraction(1, 2) else 0)
+        r = -r if v < 0 else r
+        got = int(fx[2 * i]) | (int(fx[2 * i + 1]) << 64)
+        assert got == r % (1 << 128), v
+        assert fl[i] == 0
+    fx, fl = oracle.fx_convert(np.array([np.nan, np.inf, -np.inf, 2.0 ** 39]))
+    assert list(fl) == [1, 2, 4, 8]
+
+
+def test_exact_tally_is_order_independent_and_close_to_sum(oracle):
+    rng = np.random.default_rng(9)
+    v = rng.random(100_000) * 1e-3
+    fx, fl = oracle.fx_convert(v)
+    tot = sum(int(fx[2 * i]) | (int(fx[2 * i + 1]) << 64) for i in range(len(v))) % (1 << 128)
+    exact = sum(Fraction(float(x)) for x in v)
+    assert abs(Fraction(tot, 1 << 88) - exact) <= Fraction(len(v), 1 << 89)
+
+
+# ----------------------------------------------------------- closed forms
+
+def _run64(oracle, m, sigs, ar, bank, min_w=0.0, dx=None, start=0):
+    dx = (1.0 - 0.0) / m if dx is None else dx
+    return oracle.simulate_bank(64, start, m, dx, min_w, sigs, ar, bank, nthread=1)
+
+
+def _bank64(oracle, n, x=0.0, index=0, mu=1.0, w=1.0):
+    b = np.zeros(n, dtype=oracle.P64)
+    b["seed"] = oracle.rnd_seed_chain(5127801, n)
+    b["x"], b["index"], b["mu"], b["wmc"] = x, index, mu, w
+    return b
+
+
+def test_pure_absorber_closed_form(oracle):
+    """ar = 1 => sig_i = 0 => no collisions; a normally incident particle
+    crosses every cell once and keeps exp(-sum sig_a * di) of its weight."""
+    m = 40
+    sigs = np.linspace(0.5, 3.0, m)
+    o = _run64(oracle, m, sigs, np.ones(m), _bank64(oracle, 50))
+    assert np.all(o["result"] == 1) and np.all(o["events"] == m) and np.all(o["collisions"] == 0)
+    T = np.exp(-np.sum(sigs) * (1.0 / m))
+    np.testing.assert_allclose(o["bank"]["wmc"], T, rtol=1e-13)
+    np.testing.assert_allclose(o["tally_exact"].sum() / 50, 1 - T, rtol=1e-13)
+    # cell-by-cell geometric attenuation
+    w = np.exp(-np.cumsum(np.concatenate([[0.0], sigs[:-1]])) / m)
+    np.testing.assert_allclose(o["tally_exact"] / 50, w * (1 - np.exp(-sigs / m)), rtol=1e-12)
+
+
+def test_pure_scatterer_conserves_weight(oracle):
+    m = 30
+    b = _bank64(oracle, 400, mu=0.3)
+    o = _run64(oracle, m, np.full(m, 4.0), np.zeros(m), b)
+    assert np.all(o["tally_exact"] == 0)
+    assert np.all(o["bank"]["wmc"] == 1.0)
+    assert set(np.unique(o["result"])) <= {-1, 1}
+    left = o["result"] == -1
+    assert np.all(o["bank"]["index"][left] == -1) and np.all(o["bank"]["index"][~left] == m)
+    assert o["collisions"].sum() > 0
+
+
+def test_streaming_left_through_vacuum(oracle):
+    """sig_i <= EPS and mu = -1: one crossing per cell, exit left."""
+    m = 25
+    b = _bank64(oracle, 10, x=1.0, index=m - 1, mu=-1.0)
+    o = _run64(oracle, m, np.full(m, 1e-5), np.zeros(m), b)
+    assert np.all(o["result"] == -1) and np.all(o["events"] == m)
+    assert np.all(o["bank"]["x"] == 0.0)
+
+
+def test_mu_zero_goes_right_without_scattering(oracle):
+    """mu = 0 takes the right-hand edge (src/layer.cpp:143-152); with
+    |mu| <= EPS di_edge = MAXREAL, and in vacuum di = MAXREAL too, so the
+    particle is put on the edge (di < di_edge is false)."""
+    m = 10
+    b = _bank64(oracle, 3, x=0.05, index=0, mu=0.0)
+    o = _run64(oracle, m, np.full(m, 1e-6), np.zeros(m), b)
+    assert np.all(o["result"] == 1) and np.all(o["events"] == m)
+
+
+def test_kill_discards_remaining_weight(oracle):
+    m = 20
+    b = _bank64(oracle, 500, mu=0.7)
+    sigs, ar = np.full(m, 5.0), np.full(m, 0.6)
+    o = _run64(oracle, m, sigs, ar, b, min_w=0.2)
+    killed = o["result"] == 0
+    assert killed.any()
+    assert np.all(o["bank"]["wmc"][killed] < 0.2)
+    escaped_w = o["bank"]["wmc"][~killed].sum()
+    residual = o["bank"]["wmc"][killed].sum()
+    # weight balance: initial = tallied + escaped + discarded residue
+    np.testing.assert_allclose(o["tally_exact"].sum() + escaped_w + residual, 500.0, rtol=1e-12)
+
+
+def test_history_results_do_not_depend_on_bank_order(oracle):
+    from mcgpu import configs
+    wl = configs.config1(3000)
+    perm = np.random.default_rng(1).permutation(3000)
+    a = _run64(oracle, wl.m, wl.sigs, wl.absorption_rates, wl.bank, wl.min_weight)
+    b = _run64(oracle, wl.m, wl.sigs, wl.absorption_rates, wl.bank[perm], wl.min_weight)
+    assert np.array_equal(a["bank"][perm].view(np.uint8), b["bank"].view(np.uint8))
+    assert np.array_equal(a["result"][perm], b["result"])
+    assert np.array_equal(a["fx"], b["fx"])
+    # the real_t summation differs with the order, the exact one does not
+    np.testing.assert_allclose(a["weights"], b["weights"], rtol=1e-12)
+
+
+def test_statistical_profile_of_sequential_reference(oracle):
+    """tex/data/WA_1000_1000000.out (1000 cells, 1e6 particles, default law,
+    point source at sqrt(2)/2) is a statistical pin: a 1e5-particle FP64 run
+    of the same problem agrees within Monte-Carlo noise."""
+    data = np.loadtxt(os.path.join(GOLDEN, "WA_1000_1000000.out"))
+    L = oracle.OracleLayer.decompose_domain(64, 0.0, 1.0, 0.7071067690849304, 1, 0, 1000, 100_000,
+                                            1e-12)
+    L.simulate(-1, 8)
+    prof = L.weights_absorbed / L.dx
+    np.testing.assert_allclose(np.arange(1000) * 1e-3 + 5e-4, data[:, 0], rtol=1e-4)
+    # totals agree to ~1%; the profile to a few % away from the source spike
+    assert abs(prof.sum() * 1e-3 - data[:, 1].sum() * 1e-3) / (data[:, 1].sum() * 1e-3) < 0.02
+    mask = np.abs(data[:, 0] - 0.7071) > 0.05
+    rel = np.abs(prof[mask] - data[mask, 1]) / data[mask, 1]
+    assert np.median(rel) < 0.03
+
+
+# -------------------------------------------------------- planted mistakes
+
+def _golden_equal(oracle):
+    L = oracle.OracleLayer.decompose_domain(32, 0.0, 1.0, X_INI_F32, 1, 0, 100, 100, 0.0)
+    L.simulate(-1, 1)
+    with tempfile.TemporaryDirectory() as d:
+        p = os.path.join(d, "WA.out")
+        L.dump_WA(p)
+        return open(p, "rb").read() == open(os.path.join(GOLDEN, "test_layer_target_WA.out"), "rb").read()
+
+
+def _absorber_closed_form_holds(oracle):
+    try:
+        test_pure_absorber_closed_form(oracle)
+        test_streaming_left_through_vacuum(oracle)
+        return True
+    except AssertionError:
+        return False
+
+
+@pytest.mark.parametrize("mutant", [1, 2, 3, 4, 5])
+def test_pins_catch_planted_mistakes(oracle, mutant, tmp_path):
+    """An oracle with one planted mistake (1 swapped scattered/absorbed
+    fraction, 2 wrong edge for mu < 0, 3 flipped direction draw, 4
+    attenuation with the scattering cross section, 5 RNG increment) fails at
+    least one pin: the golden file, the reference build per history, or a
+    closed form.  (The golden file alone cannot see 1 and 4: its law has
+    absorption_rates = 0.5, where sig_a == sig_i.)"""
+    M = oracle.build_mutant(mutant, str(tmp_path))
+    with oracle.using(M):
+        results = {"golden": _golden_equal(oracle), "closed_form": _absorber_closed_form_holds(oracle)}
+        if oracle.ref_available():
+            results["reference"] = all(_ref_per_history_equal(oracle, c) for c in REF_CASES)
+    assert not all(results.values()), results
+    # and the faithful build passes all of them
+    assert _golden_equal(oracle) and _absorber_closed_form_holds(oracle)
