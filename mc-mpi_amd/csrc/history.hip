@@ -22,60 +22,6 @@
 
 namespace {
 
-__device__ __forceinline__ unsigned long long readlane64(unsigned long long v, int lane) {
-  unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, lane);
-  unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), lane);
-  return ((unsigned long long)hi << 32) | lo;
-}
-
-/* number of set bits of `mask` in lanes below this one */
-__device__ __forceinline__ int mbcnt64(unsigned long long mask) {
-  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
-                                        __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
-}
-
-__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
-
-__device__ __forceinline__ void load_state(const mc_transport_args &a, int64_t i, mc_state &p) {
-  if (a.aos) {
-    const mcgpu_particle *q = a.aos + i;
-    p.seed = q->seed;
-    p.x = q->x;
-    p.mu = q->mu;
-    p.wmc = q->wmc;
-    p.index = q->index;
-  } else {
-    p.seed = a.soa.seed[i];
-    p.x = a.soa.x[i];
-    p.mu = a.soa.mu[i];
-    p.wmc = a.soa.wmc[i];
-    p.index = a.soa.index[i];
-  }
-}
-
-__device__ __forceinline__ void store_state(const mc_transport_args &a, int64_t i,
-                                            const mc_state &p, int code) {
-  if (a.aos) {
-    mcgpu_particle *q = a.aos + i;
-    q->seed = p.seed;
-    q->x = p.x;
-    q->mu = p.mu;
-    q->wmc = p.wmc;
-    q->index = p.index;
-    if (a.result32) a.result32[i] = code;
-  } else {
-    a.soa.seed[i] = p.seed;
-    a.soa.x[i] = p.x;
-    a.soa.mu[i] = p.mu;
-    a.soa.wmc[i] = p.wmc;
-    a.soa.index[i] = p.index;
-    if (a.result8) a.result8[i] = (int8_t)code;
-  }
-}
-
 template <bool LDS_TALLY, bool LDS_CELLS>
 __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refill_k) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -118,9 +64,9 @@ __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refil
         const int leader = __ffsll((long long)need) - 1;
         unsigned long long base = 0;
         if (lane == leader) base = atomicAdd(&a.ctr->next, (unsigned long long)cnt);
-        base = readlane64(base, leader);
+        base = mc_readlane64(base, leader);
         if (!has) {
-          const int64_t i = (int64_t)(base + (unsigned long long)mbcnt64(need));
+          const int64_t i = (int64_t)(base + (unsigned long long)mc_mbcnt64(need));
           if (i < a.n) {
             load_state(a, i, p);
             idx = i;
@@ -143,13 +89,7 @@ __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refil
         const double dw = mc_event(p, cell, a.L.dx, coll, di);
         ++ev;
         col += coll ? 1u : 0u;
-        uint32_t f = 0;
-        const mc_fx v = mc_fx_from_double(dw, f);
-        if (f) {
-          atomicOr(&a.flags[c], f);
-          if (f & 8u) atomicOr(&a.ctr->status, 2u);
-        }
-        mc_fx_atomic_add(&t_lo[c], &t_hi[c], v);
+        mc_tally(t_lo, t_hi, a.flags, &a.ctr->status, c, dw);
       }
       const bool alive = mc_alive(p, a.L);
       if (!alive || ev >= a.max_events) {
@@ -171,11 +111,11 @@ __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refil
     }
   }
 
-  tot_ev = wave_sum(tot_ev);
-  tot_col = wave_sum(tot_col);
-  nl = wave_sum(nl);
-  nr = wave_sum(nr);
-  nk = wave_sum(nk);
+  tot_ev = mc_wave_sum(tot_ev);
+  tot_col = mc_wave_sum(tot_col);
+  nl = mc_wave_sum(nl);
+  nr = mc_wave_sum(nr);
+  nk = mc_wave_sum(nk);
   if (lane == 0) {
     atomicAdd(&a.ctr->events, tot_ev);
     atomicAdd(&a.ctr->collisions, tot_col);

@@ -73,6 +73,44 @@ struct mc_transport_args {
   uint64_t max_events;
 };
 
+/* particle i of the call's bank (AoS staging or SoA device bank) */
+static __device__ __forceinline__ void load_state(const mc_transport_args &a, int64_t i, mc_state &p) {
+  if (a.aos) {
+    const mcgpu_particle *q = a.aos + i;
+    p.seed = q->seed;
+    p.x = q->x;
+    p.mu = q->mu;
+    p.wmc = q->wmc;
+    p.index = q->index;
+  } else {
+    p.seed = a.soa.seed[i];
+    p.x = a.soa.x[i];
+    p.mu = a.soa.mu[i];
+    p.wmc = a.soa.wmc[i];
+    p.index = a.soa.index[i];
+  }
+}
+
+static __device__ __forceinline__ void store_state(const mc_transport_args &a, int64_t i,
+                                            const mc_state &p, int code) {
+  if (a.aos) {
+    mcgpu_particle *q = a.aos + i;
+    q->seed = p.seed;
+    q->x = p.x;
+    q->mu = p.mu;
+    q->wmc = p.wmc;
+    q->index = p.index;
+    if (a.result32) a.result32[i] = code;
+  } else {
+    a.soa.seed[i] = p.seed;
+    a.soa.x[i] = p.x;
+    a.soa.mu[i] = p.mu;
+    a.soa.wmc[i] = p.wmc;
+    a.soa.index[i] = p.index;
+    if (a.result8) a.result8[i] = (int8_t)code;
+  }
+}
+
 int mc_launch_history(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_options &o,
                       hipStream_t s);
 int mc_launch_event(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_options &o,

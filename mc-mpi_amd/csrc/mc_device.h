@@ -207,8 +207,35 @@ __device__ __forceinline__ int mc_exit_code(const mc_state &p, const mc_layer_k 
 }
 
 /* -------------------------------------------------------------- wave utils */
-__device__ __forceinline__ int mc_lane() { return __lane_id(); }
+/* value of v in lane `lane` (wave-uniform lane index) */
+__device__ __forceinline__ unsigned long long mc_readlane64(unsigned long long v, int lane) {
+  unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, lane);
+  unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), lane);
+  return ((unsigned long long)hi << 32) | lo;
+}
 
-__device__ __forceinline__ unsigned long long mc_ballot(bool pred) { return __ballot(pred); }
+/* number of set bits of `mask` in lanes below this one (prefix count) */
+__device__ __forceinline__ int mc_mbcnt64(unsigned long long mask) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+__device__ __forceinline__ unsigned long long mc_wave_sum(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+/* add one tally term (exact fixed point) to cell c of (lo, hi); non-finite or
+   out-of-range terms set the cell's flag word and the call's status */
+__device__ __forceinline__ void mc_tally(unsigned long long *lo, unsigned long long *hi,
+                                         uint32_t *flags, unsigned int *status, int c, double dw) {
+  uint32_t f = 0;
+  const mc_fx v = mc_fx_from_double(dw, f);
+  if (f) {
+    atomicOr(&flags[c], f);
+    if (f & 8u) atomicOr(status, 2u);
+  }
+  mc_fx_atomic_add(&lo[c], &hi[c], v);
+}
 
 #endif /* MC_DEVICE_H */

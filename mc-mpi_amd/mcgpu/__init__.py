@@ -19,7 +19,7 @@ _KERNELS = {"history": KERNEL_HISTORY, "event": KERNEL_EVENT}
 
 
 def make_options(kernel="history", block_size=0, grid_blocks=0, events_per_refill=0,
-                 compaction_interval=0, max_events_per_history=0):
+                 compaction_interval=0, particles_per_lane=0, max_events_per_history=0):
     o = Options()
     lib().mcgpu_default_options(C.byref(o))
     o.kernel = _KERNELS[kernel] if isinstance(kernel, str) else int(kernel)
@@ -27,6 +27,7 @@ def make_options(kernel="history", block_size=0, grid_blocks=0, events_per_refil
     o.grid_blocks = grid_blocks
     o.events_per_refill = events_per_refill
     o.compaction_interval = compaction_interval
+    o.particles_per_lane = particles_per_lane
     o.max_events_per_history = max_events_per_history
     return o
 

@@ -37,7 +37,7 @@ class Bank(C.Structure):
 class Options(C.Structure):
     _fields_ = [("kernel", C.c_int32), ("block_size", C.c_int32), ("grid_blocks", C.c_int32),
                 ("events_per_refill", C.c_int32), ("compaction_interval", C.c_int32),
-                ("reserved_", C.c_int32), ("max_events_per_history", C.c_uint64)]
+                ("particles_per_lane", C.c_int32), ("max_events_per_history", C.c_uint64)]
 
 
 class Stats(C.Structure):

@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 SEQ_RTOL = 1e-12  # tallies vs the reference-order FP64 sum (summation order only)
 
-KERNELS = ["history"]
+KERNELS = ["history", "event"]
 
 
 def _say(msg):
@@ -143,9 +143,18 @@ def test_device_api_bit_exact(ctx, oracle, name, n, kernel):
     _assert_tally(w.cpu().numpy(), o)
 
 
-@pytest.mark.parametrize("opts", [dict(block_size=64), dict(block_size=1024),
-                                  dict(events_per_refill=1), dict(events_per_refill=64),
-                                  dict(grid_blocks=1), dict(grid_blocks=3, block_size=128)])
+LAUNCH_OPTS = [
+    dict(kernel="history", block_size=64), dict(kernel="history", block_size=1024),
+    dict(kernel="history", events_per_refill=1), dict(kernel="history", events_per_refill=64),
+    dict(kernel="history", grid_blocks=1), dict(kernel="history", grid_blocks=3, block_size=128),
+    dict(kernel="event", block_size=64), dict(kernel="event", block_size=1024),
+    dict(kernel="event", particles_per_lane=1), dict(kernel="event", particles_per_lane=8),
+    dict(kernel="event", compaction_interval=3), dict(kernel="event", compaction_interval=16),
+    dict(kernel="event", grid_blocks=1), dict(kernel="event", grid_blocks=5, block_size=128,
+                                              particles_per_lane=2, compaction_interval=2)]
+
+
+@pytest.mark.parametrize("opts", LAUNCH_OPTS)
 def test_launch_shape_does_not_change_results(ctx, oracle, opts):
     import mcgpu
     wl = _workload("config1", 5_000)
@@ -155,7 +164,7 @@ def test_launch_shape_does_not_change_results(ctx, oracle, opts):
     w = np.zeros(wl.m)
     result, st, rc = ctx.simulate(bank, wl.sigs, wl.absorption_rates, w, wl.min_index,
                                   wl.max_index, wl.dx, wl.min_weight,
-                                  options=mcgpu.make_options("history", **opts))
+                                  options=mcgpu.make_options(**opts))
     _assert_bank_equal(bank, o["bank"])
     assert np.array_equal(result, o["result"])
     _assert_tally(w, o)
@@ -176,8 +185,9 @@ def _layer_case(oracle, m, n, seed=0, start=0, x_mode="face"):
     return sigs, ar, dx, bank
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("m", [12_000, 5_000, 64])
-def test_tally_paths_lds_and_global(ctx, oracle, m):
+def test_tally_paths_lds_and_global(ctx, oracle, m, kernel):
     """m = 12000 exceeds one block's LDS (global 128-bit tally), 5000 keeps
     the tally in LDS with the cross sections in L2, 64 keeps both in LDS."""
     _say(f"tally path m={m}")
@@ -185,13 +195,16 @@ def test_tally_paths_lds_and_global(ctx, oracle, m):
     o = oracle.simulate_bank(64, 0, m, dx, 1e-3, sigs, ar, bank, nthread=8)
     b = bank.copy()
     w = np.zeros(m)
-    result, st, rc = ctx.simulate(b, sigs, ar, w, 0, m, dx, 1e-3)
+    import mcgpu
+    result, st, rc = ctx.simulate(b, sigs, ar, w, 0, m, dx, 1e-3,
+                                  options=mcgpu.make_options(kernel))
     _assert_bank_equal(b, o["bank"])
     assert np.array_equal(result, o["result"])
     _assert_tally(w, o)
 
 
-def test_edge_particles(ctx, oracle):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_edge_particles(ctx, oracle, kernel):
     """Particles the loop never steps (outside the layer, below the minimum
     weight, NaN weight) get the reference's exit codes; mu = 0 goes right;
     interior layer with nonzero index_start; weights accumulate (+=)."""
@@ -215,7 +228,9 @@ def test_edge_particles(ctx, oracle):
     o = oracle.simulate_bank(64, start, m, dx, 1e-3, sigs, ar, bank, nthread=1, weights=w0.copy())
     b = bank.copy()
     w = w0.copy()
-    result, st, rc = ctx.simulate(b, sigs, ar, w, start, start + m, dx, 1e-3)
+    import mcgpu
+    result, st, rc = ctx.simulate(b, sigs, ar, w, start, start + m, dx, 1e-3,
+                                  options=mcgpu.make_options(kernel))
     _assert_bank_equal(b, o["bank"])
     assert np.array_equal(result, o["result"])
     assert result[0] == -1 and result[1] == 1 and result[6] == 0 and result[3] == -1
@@ -237,14 +252,15 @@ def test_empty_bank(ctx):
     assert rc == 0 and torch.all(wd == 0.5)
 
 
-def test_event_cap_reports_error(ctx, oracle):
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_event_cap_reports_error(ctx, oracle, kernel):
     import mcgpu
     wl = _workload("config1", 500)
     b = wl.bank.copy()
     w = np.zeros(wl.m)
     result, st, rc = ctx.simulate(b, wl.sigs, wl.absorption_rates, w, wl.min_index, wl.max_index,
                                   wl.dx, wl.min_weight,
-                                  options=mcgpu.make_options(max_events_per_history=5),
+                                  options=mcgpu.make_options(kernel, max_events_per_history=5),
                                   allow=(mcgpu.ERR_EVENT_CAP,))
     assert rc == mcgpu.ERR_EVENT_CAP
     assert np.any(result == 2)
@@ -288,8 +304,9 @@ def test_repeated_calls_accumulate(ctx, oracle):
 
 # ---------------------------------------------------- full-size properties
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("name,n", [("config3", 20_000_000), ("config2", 1_000_000)])
-def test_full_size_properties_and_sampled_histories(ctx, oracle, name, n):
+def test_full_size_properties_and_sampled_histories(ctx, oracle, name, n, kernel):
     """At BASELINE sizes: counts and weight balance close exactly, every exit
     state is consistent, and a random sample of histories, re-run alone on
     the oracle from their initial states, matches bit for bit (histories are
@@ -297,14 +314,14 @@ def test_full_size_properties_and_sampled_histories(ctx, oracle, name, n):
     import torch
     import mcgpu
     wl = _workload(name, n)
-    _say(f"full size {name} n={n}")
+    _say(f"full size {name} n={n} kernel={kernel}")
     bank = mcgpu.bank_to_device(wl.bank)
     w = torch.zeros(wl.m, dtype=torch.float64, device="cuda")
     res = torch.empty(n, dtype=torch.int8, device="cuda")
     sigs = torch.from_numpy(wl.sigs).cuda()
     ar = torch.from_numpy(wl.absorption_rates).cuda()
     st, rc = ctx.simulate_device(bank, sigs, ar, w, wl.min_index, wl.max_index, wl.dx,
-                                 wl.min_weight, result=res)
+                                 wl.min_weight, result=res, options=mcgpu.make_options(kernel))
     _say(f"  events={st['events']} kernel_ms={st['kernel_ms']:.1f}")
     r = res.cpu().numpy()
     fin = mcgpu.bank_from_device(bank)
