@@ -69,8 +69,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--particles", type=int, default=100_000_000)
     ap.add_argument("--kernel", default="history")
+    ap.add_argument("--block", type=int, default=0, help="threads per block (0 = auto)")
+    ap.add_argument("--ppl", type=int, default=0, help="event kernel: pool slots per lane")
+    ap.add_argument("--ci", type=int, default=0, help="event kernel: compaction interval")
+    ap.add_argument("--refill", type=int, default=0, help="history kernel: events per refill")
     ap.add_argument("--cpu-sample", type=int, default=40_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run one 1 GiB device-to-device copy first (untimed), the "
+                         "known-size reference kernel for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported",
@@ -84,13 +91,20 @@ def main():
     n = args.particles
     wl = configs.config2(n, with_bank=False)
     ctx = mcgpu.Context(0)
+    if args.calibrate_copy:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     log(f"building the {n}-particle bank on the device")
     bank0 = device_bank(ctx, n, torch)
     bank = {k: v.clone() for k, v in bank0.items()}
     sigs = torch.from_numpy(wl.sigs).cuda()
     ar = torch.from_numpy(wl.absorption_rates).cuda()
     w = torch.zeros(wl.m, dtype=torch.float64, device="cuda")
-    opt = mcgpu.make_options(args.kernel)
+    opt = mcgpu.make_options(args.kernel, block_size=args.block, particles_per_lane=args.ppl,
+                             compaction_interval=args.ci, events_per_refill=args.refill)
 
     def step():
         for k in bank:

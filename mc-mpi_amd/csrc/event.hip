@@ -33,53 +33,88 @@ namespace {
 
 #define DEAD_ID 0xFFFFFFFFu
 
-/* one half of a block's pool, structure of arrays */
-struct pool_half {
-  uint64_t *seed;
-  double *x;
-  double *mu;
-  double *wmc;
-  int32_t *index;
-  uint32_t *id;
-  uint32_t *ev;
-  uint32_t *col;
-};
-
+/* A block's pool: two halves of P slots each, structure of arrays.  Field
+   k of slot j in half h lives at base + h*HALF + OFF_k*P + j*size_k, with
+   the 8-byte fields first.  Addresses are formed from one base pointer so
+   the compiler keeps a single scalar base per half live. */
 constexpr size_t SLOT_BYTES = 8 * 4 + 4 * 4; /* 48 bytes per slot */
 
-__device__ __forceinline__ pool_half pool_at(unsigned char *base, int P) {
-  pool_half h;
-  h.seed = reinterpret_cast<uint64_t *>(base);
-  h.x = reinterpret_cast<double *>(base + 8 * (size_t)P);
-  h.mu = reinterpret_cast<double *>(base + 16 * (size_t)P);
-  h.wmc = reinterpret_cast<double *>(base + 24 * (size_t)P);
-  h.index = reinterpret_cast<int32_t *>(base + 32 * (size_t)P);
-  h.id = reinterpret_cast<uint32_t *>(base + 36 * (size_t)P);
-  h.ev = reinterpret_cast<uint32_t *>(base + 40 * (size_t)P);
-  h.col = reinterpret_cast<uint32_t *>(base + 44 * (size_t)P);
-  return h;
+struct pool_ref {
+  unsigned char *b; /* base of one half */
+  int P;
+  __device__ __forceinline__ uint64_t *seed() const { return reinterpret_cast<uint64_t *>(b); }
+  __device__ __forceinline__ double *x() const { return reinterpret_cast<double *>(b + 8 * (size_t)P); }
+  __device__ __forceinline__ double *mu() const { return reinterpret_cast<double *>(b + 16 * (size_t)P); }
+  __device__ __forceinline__ double *wmc() const { return reinterpret_cast<double *>(b + 24 * (size_t)P); }
+  __device__ __forceinline__ int32_t *index() const { return reinterpret_cast<int32_t *>(b + 32 * (size_t)P); }
+  __device__ __forceinline__ uint32_t *id() const { return reinterpret_cast<uint32_t *>(b + 36 * (size_t)P); }
+  __device__ __forceinline__ uint32_t *ev() const { return reinterpret_cast<uint32_t *>(b + 40 * (size_t)P); }
+  __device__ __forceinline__ uint32_t *col() const { return reinterpret_cast<uint32_t *>(b + 44 * (size_t)P); }
+};
+
+/* Pool accesses use ordinary loads and stores by default: measured on
+   MI355X (config 2, 2e7 particles) non-temporal hints made the kernel 40 %
+   slower, as the pools (about 180 MB at 8 slots per lane) then no longer stay
+   in the Infinity Cache.  The hints remain selectable for experiments. */
+#ifndef MC_POOL_NT_LOAD
+#define MC_POOL_NT_LOAD 0
+#endif
+#ifndef MC_POOL_NT_STORE
+#define MC_POOL_NT_STORE 0
+#endif
+template <typename T> __device__ __forceinline__ T ld_nt(const T *p) {
+#if MC_POOL_NT_LOAD
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+template <typename T> __device__ __forceinline__ void st_nt(T *p, T v) {
+#if MC_POOL_NT_STORE
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
 }
 
-__device__ __forceinline__ void put(const pool_half &h, int j, const mc_state &p, uint32_t id,
+template <bool COLL>
+__device__ __forceinline__ void put(const pool_ref &h, int j, const mc_state &p, uint32_t id,
                                     uint32_t ev, uint32_t col) {
-  h.seed[j] = p.seed;
-  h.x[j] = p.x;
-  h.mu[j] = p.mu;
-  h.wmc[j] = p.wmc;
-  h.index[j] = p.index;
-  h.id[j] = id;
-  h.ev[j] = ev;
-  h.col[j] = col;
+  st_nt(h.seed() + j, p.seed);
+  st_nt(h.x() + j, p.x);
+  st_nt(h.mu() + j, p.mu);
+  st_nt(h.wmc() + j, p.wmc);
+  st_nt(h.index() + j, p.index);
+  st_nt(h.id() + j, id);
+  st_nt(h.ev() + j, ev);
+  if (COLL) st_nt(h.col() + j, col);
 }
 
-template <bool LDS_TALLY, bool LDS_CELLS>
+/* the registers of one pool slot */
+struct slot_regs {
+  mc_state p;
+  uint32_t id, ev, col;
+  bool valid;
+};
+
+template <bool COLL>
+__device__ __forceinline__ void load_slot(const pool_ref &h, int j, slot_regs &r) {
+  r.id = ld_nt(h.id() + j);
+  r.p.seed = ld_nt(h.seed() + j);
+  r.p.x = ld_nt(h.x() + j);
+  r.p.mu = ld_nt(h.mu() + j);
+  r.p.wmc = ld_nt(h.wmc() + j);
+  r.p.index = ld_nt(h.index() + j);
+  r.ev = ld_nt(h.ev() + j);
+  r.col = COLL ? ld_nt(h.col() + j) : 0u;
+}
+
+template <bool LDS_TALLY, bool LDS_CELLS, bool COLL>
 __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned char *pools, int P,
                                                 int compact_every) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ unsigned int s_cnt[3];
-  __shared__ unsigned long long s_base;
-  __shared__ int s_take;
-
+  /* block totals: events, collisions, left, right, killed */
+  __shared__ unsigned long long s_tot[5];
   const int m = a.L.m;
   unsigned long long *t_lo, *t_hi;
   size_t off = 0;
@@ -101,51 +136,45 @@ __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned ch
     for (int c = threadIdx.x; c < m; c += blockDim.x) lc[c] = a.cells[c];
     cells = lc;
   }
-  if (threadIdx.x == 0) {
-    s_cnt[0] = 0;
-    s_cnt[1] = 0;
-    s_cnt[2] = 0;
-  }
+  if (threadIdx.x < 5) s_tot[threadIdx.x] = 0;
+  __syncthreads();
 
-  unsigned char *mine = pools + (size_t)blockIdx.x * 2 * SLOT_BYTES * (size_t)P;
-  pool_half half[2] = {pool_at(mine, P), pool_at(mine + SLOT_BYTES * (size_t)P, P)};
-  const int B = blockDim.x;
-  const int tid = threadIdx.x;
+  /* this wave's pool: two halves of P slots */
   const int lane = __lane_id();
+  /* wave index made explicitly uniform so the pool addresses stay scalar */
+  const int wave = __builtin_amdgcn_readfirstlane(
+      (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
+  const size_t half_bytes = SLOT_BYTES * (size_t)P;
+  unsigned char *mine = pools + (size_t)wave * 2 * half_bytes;
   const uint32_t max_ev = a.max_events > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)a.max_events;
 
   int cur = 0;
-  int n_pool = 0; /* slots in use in half[cur] (block-uniform) */
+  int n_pool = 0; /* slots in use in the current half (wave-uniform) */
   bool queue_done = false;
   bool capped = false;
-  unsigned long long tot_ev = 0, tot_col = 0, nl = 0, nr = 0, nk = 0;
-  __syncthreads();
 
   for (unsigned pass = 0;; ++pass) {
-    /* ---- refill from the work queue (block-uniform condition) */
-    if (!queue_done && n_pool <= P - B) {
-      if (tid == 0) {
-        const int want = P - n_pool;
-        unsigned long long base = atomicAdd(&a.ctr->next, (unsigned long long)want);
-        int take = 0;
-        if (base < (unsigned long long)a.n) {
-          unsigned long long left = (unsigned long long)a.n - base;
-          take = left < (unsigned long long)want ? (int)left : want;
-        }
-        s_base = base;
-        s_take = take;
+    /* ---- refill from the work queue (wave-uniform condition) */
+    if (!queue_done && n_pool <= P - 64) {
+      const int want = P - n_pool;
+      unsigned long long base = 0;
+      if (lane == 0) base = atomicAdd(&a.ctr->next, (unsigned long long)want);
+      base = mc_readlane64(base, 0);
+      int take = 0;
+      if (base < (unsigned long long)a.n) {
+        const unsigned long long left = (unsigned long long)a.n - base;
+        take = left < (unsigned long long)want ? (int)left : want;
       }
-      __syncthreads();
-      const unsigned long long base = s_base;
-      const int take = s_take;
-      if (take < P - n_pool) queue_done = true;
-      for (int j = tid; j < take; j += B) {
+      if (take < want) queue_done = true;
+      const pool_ref h{mine + cur * half_bytes, P};
+      for (int k = lane; k < take; k += 64) {
         mc_state p;
-        load_state(a, (int64_t)(base + j), p);
-        put(half[cur], n_pool + j, p, (uint32_t)(base + j), 0u, 0u);
+        load_state(a, (int64_t)(base + k), p);
+        put<COLL>(h, n_pool + k, p, (uint32_t)(base + k), 0u, 0u);
       }
       n_pool += take;
-      __syncthreads();
+      /* slots written by one lane are read by another lane of this wave */
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     }
     if (n_pool == 0) {
       if (queue_done) break;
@@ -154,28 +183,25 @@ __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned ch
 
     /* ---- one event per live particle */
     const bool compact = (pass % (unsigned)compact_every) == 0;
-    unsigned int *cnt = &s_cnt[pass % 3];
-    if (tid == 0) s_cnt[(pass + 1) % 3] = 0;
-    const pool_half src = half[cur];
-    const pool_half dst = half[cur ^ 1];
-    for (int j0 = 0; j0 < n_pool; j0 += B) { /* block-uniform trip count */
-      const int j = j0 + tid;
-      bool live = false, keep = false;
-      mc_state p;
-      uint32_t id = DEAD_ID, ev = 0, col = 0;
-      if (j < n_pool) {
-        id = src.id[j];
-        if (id != DEAD_ID) {
-          p.seed = src.seed[j];
-          p.x = src.x[j];
-          p.mu = src.mu[j];
-          p.wmc = src.wmc[j];
-          p.index = src.index[j];
-          ev = src.ev[j];
-          col = src.col[j];
-          live = true;
-        }
+    const pool_ref src{mine + cur * half_bytes, P};
+    const pool_ref dst{mine + (cur ^ 1) * half_bytes, P};
+    int wpos = 0; /* wave-uniform output position of the compaction */
+    unsigned int pass_ev = 0, pass_col = 0; /* this lane's events of the pass */
+    /* software pipeline: the next slot is loaded while this one's event is
+       computed, so its memory latency overlaps the FP64 work */
+    slot_regs nxt;
+    nxt.valid = lane < n_pool;
+    if (nxt.valid) load_slot<COLL>(src, lane, nxt);
+    for (int j0 = 0; j0 < n_pool; j0 += 64) { /* wave-uniform trip count */
+      const int j = j0 + lane;
+      mc_state p = nxt.p;
+      uint32_t id = nxt.valid ? nxt.id : DEAD_ID, ev = nxt.ev, col = nxt.col;
+      if (j0 + 64 < n_pool) {
+        nxt.valid = j + 64 < n_pool;
+        if (nxt.valid) load_slot<COLL>(src, j + 64, nxt);
       }
+      const bool live = id != DEAD_ID;
+      bool keep = false;
       if (live) {
         if (mc_alive(p, a.L) && ev < max_ev) {
           const int c = p.index - a.L.index_start;
@@ -183,7 +209,9 @@ __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned ch
           double di;
           const double dw = mc_event(p, cells[c], a.L.dx, coll, di);
           ++ev;
+          ++pass_ev;
           col += coll ? 1u : 0u;
+          pass_col += coll ? 1u : 0u;
           mc_tally(t_lo, t_hi, a.flags, &a.ctr->status, c, dw);
         }
         keep = mc_alive(p, a.L) && ev < max_ev;
@@ -196,53 +224,48 @@ __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned ch
           store_state(a, (int64_t)id, p, code);
           if (a.events) a.events[id] = ev;
           if (a.collisions) a.collisions[id] = col;
-          tot_ev += ev;
-          tot_col += col;
-          nl += code == -1;
-          nr += code == 1;
-          nk += code == 0;
+          atomicAdd(&s_tot[code == -1 ? 2 : (code == 1 ? 3 : 4)], code == 2 ? 0ull : 1ull);
         }
       }
       if (compact) {
+        /* stream compaction: survivors keep their order, ranks from the ballot */
         const unsigned long long mask = __ballot(keep);
-        if (mask) {
-          const int leader = __ffsll((long long)mask) - 1;
-          unsigned int wbase = 0;
-          if (lane == leader) wbase = atomicAdd(cnt, (unsigned int)__popcll(mask));
-          wbase = (unsigned int)__builtin_amdgcn_readlane((int)wbase, leader);
-          if (keep) put(dst, (int)wbase + mc_mbcnt64(mask), p, id, ev, col);
-        }
+        if (keep) put<COLL>(dst, wpos + mc_mbcnt64(mask), p, id, ev, col);
+        wpos += __popcll(mask);
       } else if (live) {
         if (keep)
-          put(src, j, p, id, ev, col);
+          put<COLL>(src, j, p, id, ev, col);
         else
-          src.id[j] = DEAD_ID;
+          st_nt(src.id() + j, DEAD_ID);
       }
     }
-    __syncthreads();
+    {
+      const unsigned long long e = mc_wave_sum(pass_ev), cl = mc_wave_sum(pass_col);
+      if (lane == 0) {
+        atomicAdd(&s_tot[0], e);
+        atomicAdd(&s_tot[1], cl);
+      }
+    }
     if (compact) {
-      n_pool = (int)*cnt;
+      n_pool = wpos;
       cur ^= 1;
+      /* compacted slots move between lanes: complete the stores first */
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     }
   }
 
-  tot_ev = mc_wave_sum(tot_ev);
-  tot_col = mc_wave_sum(tot_col);
-  nl = mc_wave_sum(nl);
-  nr = mc_wave_sum(nr);
-  nk = mc_wave_sum(nk);
-  if (lane == 0) {
-    atomicAdd(&a.ctr->events, tot_ev);
-    atomicAdd(&a.ctr->collisions, tot_col);
-    atomicAdd(&a.ctr->n_left, nl);
-    atomicAdd(&a.ctr->n_right, nr);
-    atomicAdd(&a.ctr->n_killed, nk);
-  }
   if (capped) atomicOr(&a.ctr->status, 1u);
 
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    atomicAdd(&a.ctr->events, s_tot[0]);
+    atomicAdd(&a.ctr->collisions, s_tot[1]);
+    atomicAdd(&a.ctr->n_left, s_tot[2]);
+    atomicAdd(&a.ctr->n_right, s_tot[3]);
+    atomicAdd(&a.ctr->n_killed, s_tot[4]);
+  }
   if (LDS_TALLY) {
-    __syncthreads();
-    for (int c = tid; c < m; c += B) {
+    for (int c = threadIdx.x; c < m; c += blockDim.x) {
       mc_fx v;
       v.lo = t_lo[c];
       v.hi = t_hi[c];
@@ -251,10 +274,10 @@ __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned ch
   }
 }
 
-template <bool T, bool C>
+template <bool T, bool C, bool COLL>
 int launch_one(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_options &o,
                hipStream_t s, size_t lds) {
-  auto kern = k_event<T, C>;
+  auto kern = k_event<T, C, COLL>;
   if (lds > 65536)
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -268,16 +291,17 @@ int launch_one(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_optio
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     grid = per_cu * ctx->num_cus;
   }
-  /* particles per lane: slots of the pool per thread */
-  const int ppl = o.particles_per_lane > 0 ? o.particles_per_lane : 4;
-  const int P = ppl * block;
+  /* particles per lane: each wave's pool holds 64 * ppl slots */
+  const int ppl = o.particles_per_lane > 0 ? o.particles_per_lane : 8;
+  const int P = ppl * 64;
   int64_t per_block = (a.n + grid - 1) / grid;
   if (grid > 1 && per_block < block) { /* small banks: fewer blocks */
     grid = (int)((a.n + block - 1) / block);
     if (grid < 1) grid = 1;
   }
   const int c = o.compaction_interval > 0 ? o.compaction_interval : 1;
-  int rc = mc_ensure(ctx, &ctx->work, &ctx->work_bytes, (size_t)grid * 2 * SLOT_BYTES * (size_t)P);
+  const size_t waves = (size_t)grid * (size_t)(block / 64);
+  int rc = mc_ensure(ctx, &ctx->work, &ctx->work_bytes, waves * 2 * SLOT_BYTES * (size_t)P);
   if (rc) return rc;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, s, a, (unsigned char *)ctx->work, P, c);
   MC_HIP(ctx, hipGetLastError());
@@ -292,10 +316,16 @@ int mc_launch_event(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_
     return mc_fail(ctx, MCGPU_ERR_ARG, "event kernel: bank larger than 2^32-2 particles");
   const size_t m = (size_t)a.L.m;
   const size_t tally = 16 * m, cells = sizeof(mc_cell) * m;
-  const size_t statics = 64; /* s_cnt, s_base, s_take */
-  const bool lds_tally = tally + statics <= ctx->lds_per_block;
+  const bool lds_tally = tally <= ctx->lds_per_block;
   const bool lds_cells = lds_tally && tally + cells <= 65536;
-  if (lds_tally && lds_cells) return launch_one<true, true>(ctx, a, o, s, tally + cells);
-  if (lds_tally) return launch_one<true, false>(ctx, a, o, s, tally);
-  return launch_one<false, false>(ctx, a, o, s, 0);
+  /* collision counters ride in the pool only when the caller asked for them
+     per particle; the call's total is kept either way */
+  if (a.collisions) {
+    if (lds_tally && lds_cells) return launch_one<true, true, true>(ctx, a, o, s, tally + cells);
+    if (lds_tally) return launch_one<true, false, true>(ctx, a, o, s, tally);
+    return launch_one<false, false, true>(ctx, a, o, s, 0);
+  }
+  if (lds_tally && lds_cells) return launch_one<true, true, false>(ctx, a, o, s, tally + cells);
+  if (lds_tally) return launch_one<true, false, false>(ctx, a, o, s, tally);
+  return launch_one<false, false, false>(ctx, a, o, s, 0);
 }

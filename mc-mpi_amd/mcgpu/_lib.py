@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_ROOT, "libmcgpu.so")
+# MCGPU_LIB selects another build of the library (e.g. for A/B timing)
+LIB_PATH = os.environ.get("MCGPU_LIB") or os.path.join(PKG_ROOT, "libmcgpu.so")
 
 OK = 0
 ERR_ARG = -1
