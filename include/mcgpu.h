@@ -149,6 +149,53 @@ int mcgpu_simulate_device(mcgpu_context *ctx, int64_t n, const mcgpu_bank *bank,
                           uint64_t *events, uint64_t *collisions, const mcgpu_options *opt,
                           mcgpu_stats *stats, void *stream);
 
+/* ------------------------------------------------------------- batched
+ * Many independent slab problems in one persistent launch (a work queue of
+ * (slab, particle range) tiles), replayed from a captured hipGraph.  Each
+ * slab is what one mcgpu_simulate_device call would transport; results are
+ * bit-identical to running the slabs one call at a time.
+ *
+ * Inputs are concatenations in device memory: slab s owns cells
+ * [cell_offset, cell_offset + max_index - min_index) of sigs, absorption_rates
+ * and weights_absorbed, and particles [particle_offset, particle_offset + n)
+ * of the bank (and of result, if given).
+ */
+typedef struct mcgpu_slab {
+  int32_t min_index;       /* the slab's global cell range [min_index, max_index) */
+  int32_t max_index;
+  double dx;
+  double particle_min_weight;
+  int64_t cell_offset;
+  int64_t particle_offset;
+  int64_t n;
+} mcgpu_slab;
+
+typedef struct mcgpu_batch mcgpu_batch;
+
+/* Build a plan: copies the slab table to the device, orders the tiles
+   (largest work first) and captures setup + transport + finalize into a
+   hipGraph.  opt: block_size, grid_blocks, events_per_refill and
+   particles_per_lane (tile = particles_per_lane * block_size particles). */
+int mcgpu_batch_create(mcgpu_context *ctx, int32_t n_slabs, const mcgpu_slab *slabs,
+                       const double *sigs, const double *absorption_rates,
+                       double *weights_absorbed, int64_t total_cells, const mcgpu_bank *bank,
+                       int8_t *result, const mcgpu_options *opt, mcgpu_batch **out);
+/* Launch the captured graph on `stream` and wait; stats may be NULL. */
+int mcgpu_batch_run(mcgpu_batch *batch, void *stream, mcgpu_stats *stats);
+int mcgpu_batch_destroy(mcgpu_batch *batch);
+/* The options a plan was built with (after autotuning, the chosen ones). */
+int mcgpu_batch_options(const mcgpu_batch *batch, mcgpu_options *opt);
+/*
+ * Autotuner: times the plan's transport for each candidate launch shape
+ * (block size x particles per lane x events per refill, up to n_candidates
+ * of the built-in list) on the plan's own inputs, restoring the bank and
+ * weights from a device copy between trials, then re-captures the graph with
+ * the fastest.  On return the bank and weights hold their original values.
+ * times_ms (may be NULL) receives each candidate's time.
+ */
+int mcgpu_batch_autotune(mcgpu_batch *batch, int32_t n_candidates, double *times_ms,
+                         mcgpu_options *chosen);
+
 /* ------------------------------------------------------------- RNG helpers
  * Known-answer helpers for the reference RNG (src/random.cpp), evaluated on
  * the device (all pointers are device pointers):

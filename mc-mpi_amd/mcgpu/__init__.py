@@ -9,11 +9,11 @@ import ctypes as C
 import numpy as np
 
 from ._lib import (ERR_ARG, ERR_EVENT_CAP, ERR_HIP, ERR_NO_DEVICE, ERR_NOMEM, ERR_RANGE,
-                   KERNEL_EVENT, KERNEL_HISTORY, OK, PARTICLE, Bank, McgpuError, Options, Stats,
-                   lib, status_string)
+                   KERNEL_EVENT, KERNEL_HISTORY, OK, PARTICLE, Bank, McgpuError, Options, Slab,
+                   Stats, lib, status_string)
 
-__all__ = ["Context", "PARTICLE", "McgpuError", "KERNEL_HISTORY", "KERNEL_EVENT", "lib",
-           "bank_to_device", "bank_from_device", "make_options"]
+__all__ = ["Context", "Batch", "PARTICLE", "McgpuError", "KERNEL_HISTORY", "KERNEL_EVENT", "lib",
+           "bank_to_device", "bank_from_device", "make_options", "options_dict"]
 
 _KERNELS = {"history": KERNEL_HISTORY, "event": KERNEL_EVENT}
 
@@ -30,6 +30,10 @@ def make_options(kernel="history", block_size=0, grid_blocks=0, events_per_refil
     o.particles_per_lane = particles_per_lane
     o.max_events_per_history = max_events_per_history
     return o
+
+
+def options_dict(o):
+    return {k: getattr(o, k) for k, _ in o._fields_}
 
 
 def _stream(stream):
@@ -167,3 +171,61 @@ class Context:
         w = {"log": 0, "exp": 1}[which]
         self.check(lib().mcgpu_math_eval(self._h, w, x.numel(), _ptr(x), _ptr(out),
                                          _stream(stream)))
+
+
+class Batch:
+    """A batched plan (mcgpu_batch_*): many independent slabs transported by
+    one persistent launch replayed from a captured hipGraph.
+
+    slabs: sequence of (min_index, max_index, dx, particle_min_weight,
+    cell_offset, particle_offset, n); sigs / absorption_rates / weights: the
+    concatenated per-cell device arrays; bank: concatenated device bank."""
+
+    def __init__(self, ctx, slabs, sigs, absorption_rates, weights, bank, result=None,
+                 options=None):
+        self.ctx = ctx
+        arr = (Slab * len(slabs))(*[Slab(*s) for s in slabs])
+        self._keep = (sigs, absorption_rates, weights, bank, result)
+        b = Bank(_ptr(bank["seed"]), _ptr(bank["x"]), _ptr(bank["mu"]), _ptr(bank["wmc"]),
+                 _ptr(bank["index"]))
+        self._h = C.c_void_p()
+        opt = options if options is not None else make_options()
+        # the plan runs on its own stream: finish the torch work that built the inputs
+        try:
+            import torch
+            torch.cuda.synchronize()
+        except ImportError:
+            pass
+        rc = lib().mcgpu_batch_create(ctx._h, len(slabs), C.cast(arr, C.c_void_p), _ptr(sigs),
+                                      _ptr(absorption_rates), _ptr(weights), weights.numel(),
+                                      C.byref(b), _ptr(result), C.byref(opt), C.byref(self._h))
+        ctx.check(rc)
+
+    def run(self, stream=None, allow=()):
+        st = Stats()
+        rc = lib().mcgpu_batch_run(self._h, _stream(stream), C.byref(st))
+        self.ctx.check(rc, allow)
+        return st.as_dict()
+
+    def autotune(self, n_candidates=0):
+        times = (C.c_double * 16)()
+        chosen = Options()
+        self.ctx.check(lib().mcgpu_batch_autotune(self._h, n_candidates, times, C.byref(chosen)))
+        n = n_candidates if n_candidates > 0 else 8
+        return [times[i] for i in range(n)], options_dict(chosen)
+
+    def options(self):
+        o = Options()
+        self.ctx.check(lib().mcgpu_batch_options(self._h, C.byref(o)))
+        return options_dict(o)
+
+    def close(self):
+        if self._h:
+            lib().mcgpu_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -49,6 +49,12 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class Slab(C.Structure):
+    _fields_ = [("min_index", C.c_int32), ("max_index", C.c_int32), ("dx", C.c_double),
+                ("particle_min_weight", C.c_double), ("cell_offset", C.c_int64),
+                ("particle_offset", C.c_int64), ("n", C.c_int64)]
+
+
 class McgpuError(RuntimeError):
     def __init__(self, status, message):
         super().__init__(f"mcgpu status {status}: {message}")
@@ -70,6 +76,12 @@ SIGNATURES = {
                              C.POINTER(Options), C.POINTER(Stats)]),
     "mcgpu_simulate_device": (i32, [vp, i64, C.POINTER(Bank), vp, vp, vp, i32, i32, dbl, dbl,
                                     vp, vp, vp, C.POINTER(Options), C.POINTER(Stats), vp]),
+    "mcgpu_batch_create": (i32, [vp, i32, vp, vp, vp, vp, i64, C.POINTER(Bank), vp,
+                                 C.POINTER(Options), C.POINTER(vp)]),
+    "mcgpu_batch_run": (i32, [vp, vp, C.POINTER(Stats)]),
+    "mcgpu_batch_destroy": (i32, [vp]),
+    "mcgpu_batch_options": (i32, [vp, C.POINTER(Options)]),
+    "mcgpu_batch_autotune": (i32, [vp, i32, vp, C.POINTER(Options)]),
     "mcgpu_rng_real_step": (i32, [vp, i64, vp, vp, vp]),
     "mcgpu_rng_seed_chain": (i32, [vp, u64, i64, vp, vp]),
     "mcgpu_math_eval": (i32, [vp, C.c_int, i64, vp, vp, vp]),

@@ -190,21 +190,40 @@ def config5(n=10_000_000, seed=5):
     return wl
 
 
-def config4_slabs(n_slabs=4096, seed=4, particle_scale=1.0, max_particles=1_000_000):
+def config4_layout(n_slabs=4096, seed=4, particle_scale=1.0):
     """4096 independent slabs on [0,1]: cells log-uniform in [32, 8192],
-    particles log-uniform in [1e3, 1e6] (times particle_scale), per cell
-    sigma_t ~ LogNormal(0, 1) and scattering ratio c ~ U(0.1, 0.99), stored as
-    sigs = sigma_t, ar = 1 - c; source at x = 0 with mu = sqrt(U), weight 1;
-    minimum weight 1e-3.  Returns a list of (m, n, sigs, ar) and the seed
-    offset of each slab's bank in the global seed chain."""
+    particles log-uniform in [1e3, 1e6] (times particle_scale, at least 1),
+    per cell sigma_t ~ LogNormal(0, 1) and scattering ratio c ~ U(0.1, 0.99)
+    stored as sigs = sigma_t, ar = 1 - c; source at x = 0, mu = sqrt(U),
+    weight 1; minimum weight 1e-3.  Slab s owns cells [cell_offset, +m) of the
+    concatenated sigs/ar and particles [particle_offset, +n) of the
+    concatenated bank, whose seeds are outputs particle_offset+1.. of the
+    reference seed chain.  Returns (slabs, sigs, ar, total_particles) with
+    slabs as tuples (min_index, max_index, dx, min_weight, cell_offset,
+    particle_offset, n)."""
     u = uniform(seed * 1000 + 1, 2 * n_slabs)
-    ms = np.floor(np.exp(np.log(32.0) + u[0::2] * (np.log(8192.0) - np.log(32.0)))).astype(int)
+    ms = np.floor(np.exp(np.log(32.0) + u[0::2] * (np.log(8192.0) - np.log(32.0)))).astype(np.int64)
     ns = np.floor(np.exp(np.log(1e3) + u[1::2] * (np.log(1e6) - np.log(1e3))) * particle_scale)
-    ns = np.clip(ns, 1, max_particles).astype(int)
+    ns = np.maximum(ns, 1).astype(np.int64)
+    total_cells = int(ms.sum())
+    st = lognormal_cells(seed * 1000 + 2, total_cells, 0.0, 1.0)
+    c = 0.1 + 0.89 * uniform(seed * 1000 + 3, total_cells)
     slabs = []
+    coff = poff = 0
     for s in range(n_slabs):
-        m = int(ms[s])
-        st = lognormal_cells(seed * 1000 + 10 + 3 * s, m, 0.0, 1.0)
-        c = 0.1 + 0.89 * uniform(seed * 1000 + 11 + 3 * s, m)
-        slabs.append((m, int(ns[s]), st, 1.0 - c))
-    return slabs
+        m, n = int(ms[s]), int(ns[s])
+        slabs.append((0, m, 1.0 / m, 1e-3, coff, poff, n))
+        coff += m
+        poff += n
+    return slabs, st, 1.0 - c, poff
+
+
+def config4_mu(n, offset=0, seed=4):
+    """Directions of config 4's source particles offset .. offset+n-1."""
+    return np.sqrt(uniform(seed * 1000 + 4, n, offset))
+
+
+def config4_bank(total_particles, offset=0, count=None, seed=4):
+    """Host bank of config 4's particles offset .. offset+count-1."""
+    count = total_particles - offset if count is None else count
+    return _bank(count, 0.0, 0, config4_mu(count, offset, seed), 1.0, seed_start=offset)

@@ -1,0 +1,59 @@
+"""C++ host side (mcgpu::Layer over the C ABI): the compiled C++ tests under
+tests/cpp, and the mcgpu_run driver."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BUILD = os.path.join(ROOT, "tests", "cpp", "_build")
+
+
+def _run(args, cwd, timeout=600):
+    out = subprocess.run(args, cwd=cwd, capture_output=True, text=True, timeout=timeout)
+    print(out.stdout[-3000:], out.stderr[-3000:], flush=True)
+    return out
+
+
+def test_yaml_loader(tmp_path):
+    exe = os.path.join(BUILD, "test_yaml")
+    assert os.path.exists(exe), "build first (__graft_entry__.build())"
+    out = _run([exe, str(tmp_path / "config.yaml")], str(tmp_path))
+    assert out.returncode == 0 and "all ok" in out.stdout
+
+
+@pytest.mark.gpu
+def test_layer_against_oracle(tmp_path):
+    """mcgpu::Layer (GPU) vs the oracle's FP64 Layer: decompose_domain,
+    lazy creation, simulate(n) sequences, escape buffers, nb_disabled,
+    weights, WA.out -- for both kernels."""
+    exe = os.path.join(BUILD, "test_layer_gpu")
+    assert os.path.exists(exe)
+    out = _run([exe], str(tmp_path))
+    assert out.returncode == 0 and "all ok" in out.stdout
+
+
+@pytest.mark.gpu
+def test_mcgpu_run_cli_on_reference_config(tmp_path):
+    """The reference's config.yaml (1000 cells, 1e5 particles, default law)
+    through mcgpu_run: every particle is disabled, WA.out and weights.csv are
+    written, and the absorbed fraction matches the oracle's FP64 Layer."""
+    import numpy as np
+    cfg = tmp_path / "config.yaml"
+    cfg.write_text("x_min: 0.000000000000000000e+00\nx_max: 1.000000000000000000e+00\n"
+                   "x_ini: 7.071067690849304199e-01\nparticle_min_weight: 9.999999960041972003e-13\n"
+                   "nb_cells: 1000\nnb_particles: 20000\ncycle_time: 1.e-05\n"
+                   "nb_particles_per_cycle: 5000\nnthread: 1\nstatistics_cycle_time: 1.e-1\n")
+    out = _run([os.path.join(ROOT, "mc-mpi_amd", "mcgpu_run"), str(cfg)], str(tmp_path))
+    assert out.returncode == 0
+    assert "disabled 20000" in out.stderr
+    wa = np.loadtxt(tmp_path / "WA.out")
+    w = np.loadtxt(tmp_path / "weights.csv", delimiter=",", skiprows=1)
+    assert wa.shape == (1000, 2) and w.shape == (1000, 3)
+    import pyoracle
+    L = pyoracle.OracleLayer.decompose_domain(64, 0.0, 1.0, 7.071067690849304199e-01, 1, 0, 1000,
+                                              20000, 9.999999960041972003e-13)
+    L.set_tally_exact(True)
+    while L.nb_disabled < 20000:
+        L.simulate(5000, 8)
+    np.testing.assert_array_equal(w[:, 2], L.weights_absorbed / L.dx)
