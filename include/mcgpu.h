@@ -196,6 +196,39 @@ int mcgpu_batch_options(const mcgpu_batch *batch, mcgpu_options *opt);
 int mcgpu_batch_autotune(mcgpu_batch *batch, int32_t n_candidates, double *times_ms,
                          mcgpu_options *chosen);
 
+/* ------------------------------------------------------------- adjoint
+ * Gradient of a weighted tally T = sum_c g[c] * tally[c] with respect to the
+ * per-cell cross sections the transport step uses, sig_a = sigs * ar and
+ * sig_i = sigs * (1.0 - ar) (src/layer.cpp:131-133).  These entry points take
+ * the two cell arrays directly (instead of sigs / absorption_rates), so a
+ * gradient and a finite difference refer to exactly the same parameters.
+ * No counterpart exists in the reference.
+ *
+ * mcgpu_forward_cells: mcgpu_simulate_device with (sig_a, sig_i) given per
+ *   cell; with g and history_score (device, n doubles) non-NULL it also
+ *   stores each history's T_h = sum over its events of g[c] * dw (history
+ *   kernel only).
+ * mcgpu_adjoint_cells: replays every history of `initial` (the bank as it was
+ *   before the forward call; not modified) and adds dT/dsig_a and dT/dsig_i
+ *   to grad_sig_a / grad_sig_i (device, m doubles, +=): the pathwise
+ *   derivative for sig_a and the score-function derivative for sig_i (each
+ *   event's term weighted by the target tally from that event on).  With
+ *   history_score NULL each history is replayed twice (score, then gradient).
+ *   The weight kill at particle_min_weight is not differentiated.
+ */
+int mcgpu_forward_cells(mcgpu_context *ctx, int64_t n, const mcgpu_bank *bank,
+                        const double *sig_a, const double *sig_i, double *weights_absorbed,
+                        int32_t min_index, int32_t max_index, double dx,
+                        double particle_min_weight, const double *g, double *history_score,
+                        int8_t *result, const mcgpu_options *opt, mcgpu_stats *stats,
+                        void *stream);
+int mcgpu_adjoint_cells(mcgpu_context *ctx, int64_t n, const mcgpu_bank *initial,
+                        const double *sig_a, const double *sig_i, int32_t min_index,
+                        int32_t max_index, double dx, double particle_min_weight,
+                        const double *g, const double *history_score, double *grad_sig_a,
+                        double *grad_sig_i, const mcgpu_options *opt, mcgpu_stats *stats,
+                        void *stream);
+
 /* ------------------------------------------------------------- RNG helpers
  * Known-answer helpers for the reference RNG (src/random.cpp), evaluated on
  * the device (all pointers are device pointers):

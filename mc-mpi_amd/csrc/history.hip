@@ -22,7 +22,7 @@
 
 namespace {
 
-template <bool LDS_TALLY, bool LDS_CELLS>
+template <bool LDS_TALLY, bool LDS_CELLS, bool SCORE>
 __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refill_k) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int m = a.L.m;
@@ -41,10 +41,16 @@ __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refil
     t_hi = a.fx + m;
   }
   const mc_cell *cells = a.cells;
+  const double *g = a.g;
   if (LDS_CELLS) {
     mc_cell *lc = reinterpret_cast<mc_cell *>(smem + off);
     for (int c = threadIdx.x; c < m; c += blockDim.x) lc[c] = a.cells[c];
     cells = lc;
+    if (SCORE) {
+      double *lg = reinterpret_cast<double *>(smem + off + sizeof(mc_cell) * (size_t)m);
+      for (int c = threadIdx.x; c < m; c += blockDim.x) lg[c] = a.g[c];
+      g = lg;
+    }
   }
   __syncthreads();
 
@@ -54,6 +60,7 @@ __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refil
   int64_t idx = 0;
   bool has = false, done = false, capped = false;
   uint64_t ev = 0, col = 0;
+  double score = 0.0;
   unsigned long long tot_ev = 0, tot_col = 0, nl = 0, nr = 0, nk = 0;
 
   for (;;) {
@@ -73,6 +80,7 @@ __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refil
             has = true;
             ev = 0;
             col = 0;
+            score = 0.0;
           }
         }
         if (base + (unsigned long long)cnt >= (unsigned long long)a.n) done = true;
@@ -90,6 +98,7 @@ __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refil
         ++ev;
         col += coll ? 1u : 0u;
         mc_tally(t_lo, t_hi, a.flags, &a.ctr->status, c, dw);
+        if (SCORE) score = score + g[c] * dw;
       }
       const bool alive = mc_alive(p, a.L);
       if (!alive || ev >= a.max_events) {
@@ -101,6 +110,7 @@ __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refil
         store_state(a, idx, p, code);
         if (a.events) a.events[idx] = ev;
         if (a.collisions) a.collisions[idx] = col;
+        if (SCORE) a.score[idx] = score;
         tot_ev += ev;
         tot_col += col;
         nl += code == -1;
@@ -136,10 +146,10 @@ __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refil
   }
 }
 
-template <bool T, bool C>
+template <bool T, bool C, bool S>
 int launch_one(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_options &o,
                hipStream_t s, size_t lds) {
-  auto kern = k_history<T, C>;
+  auto kern = k_history<T, C, S>;
   if (lds > 65536)
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -169,8 +179,16 @@ int mc_launch_history(mcgpu_context *ctx, const mc_transport_args &a, const mcgp
   const size_t m = (size_t)a.L.m;
   const size_t tally = 16 * m, cells = sizeof(mc_cell) * m;
   const bool lds_tally = tally <= ctx->lds_per_block;
+  if (a.score) { /* per-history score of a weighted tally (forward of the adjoint) */
+    if (!a.g) return mc_fail(ctx, MCGPU_ERR_ARG, "score output needs target weights g");
+    const size_t extra = 8 * m;
+    const bool lds_cells = lds_tally && tally + cells + extra <= 65536;
+    if (lds_tally && lds_cells) return launch_one<true, true, true>(ctx, a, o, s, tally + cells + extra);
+    if (lds_tally) return launch_one<true, false, true>(ctx, a, o, s, tally);
+    return launch_one<false, false, true>(ctx, a, o, s, 0);
+  }
   const bool lds_cells = lds_tally && tally + cells <= 65536;
-  if (lds_tally && lds_cells) return launch_one<true, true>(ctx, a, o, s, tally + cells);
-  if (lds_tally) return launch_one<true, false>(ctx, a, o, s, tally);
-  return launch_one<false, false>(ctx, a, o, s, 0);
+  if (lds_tally && lds_cells) return launch_one<true, true, false>(ctx, a, o, s, tally + cells);
+  if (lds_tally) return launch_one<true, false, false>(ctx, a, o, s, tally);
+  return launch_one<false, false, false>(ctx, a, o, s, 0);
 }

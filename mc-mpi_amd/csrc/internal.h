@@ -71,6 +71,8 @@ struct mc_transport_args {
   uint64_t *collisions;
   mc_counters *ctr;
   uint64_t max_events;
+  const double *g; /* optional per-cell target weights (history kernel) */
+  double *score;   /* optional per-history sum of g[c] * dw, in event order */
 };
 
 /* particle i of the call's bank (AoS staging or SoA device bank) */

@@ -158,6 +158,39 @@ class Context:
         self.check(rc, allow)
         return st.as_dict(), rc
 
+    # ------------------------------------------------------------ adjoint
+    def forward_cells(self, bank, sig_a, sig_i, weights_absorbed, min_index, max_index, dx,
+                      particle_min_weight, g=None, history_score=None, result=None,
+                      options=None, stream=None):
+        """mcgpu_forward_cells: transport with per-cell (sig_a, sig_i) given
+        directly; optionally stores each history's weighted tally."""
+        b = Bank(_ptr(bank["seed"]), _ptr(bank["x"]), _ptr(bank["mu"]), _ptr(bank["wmc"]),
+                 _ptr(bank["index"]))
+        st = Stats()
+        opt = options if options is not None else make_options()
+        rc = lib().mcgpu_forward_cells(self._h, bank["x"].numel(), C.byref(b), _ptr(sig_a),
+                                       _ptr(sig_i), _ptr(weights_absorbed), min_index, max_index,
+                                       dx, particle_min_weight, _ptr(g), _ptr(history_score),
+                                       _ptr(result), C.byref(opt), C.byref(st), _stream(stream))
+        self.check(rc)
+        return st.as_dict()
+
+    def adjoint_cells(self, initial_bank, sig_a, sig_i, min_index, max_index, dx,
+                      particle_min_weight, g, grad_sig_a, grad_sig_i, history_score=None,
+                      options=None, stream=None):
+        """mcgpu_adjoint_cells: adds dT/dsig_a, dT/dsig_i of T = sum g*tally."""
+        b = Bank(_ptr(initial_bank["seed"]), _ptr(initial_bank["x"]), _ptr(initial_bank["mu"]),
+                 _ptr(initial_bank["wmc"]), _ptr(initial_bank["index"]))
+        st = Stats()
+        opt = options if options is not None else make_options()
+        rc = lib().mcgpu_adjoint_cells(self._h, initial_bank["x"].numel(), C.byref(b), _ptr(sig_a),
+                                       _ptr(sig_i), min_index, max_index, dx, particle_min_weight,
+                                       _ptr(g), _ptr(history_score), _ptr(grad_sig_a),
+                                       _ptr(grad_sig_i), C.byref(opt), C.byref(st),
+                                       _stream(stream))
+        self.check(rc)
+        return st.as_dict()
+
     # ------------------------------------------------------------ helpers
     def rng_real_step(self, seeds, reals, stream=None):
         self.check(lib().mcgpu_rng_real_step(self._h, seeds.numel(), _ptr(seeds), _ptr(reals),

@@ -82,6 +82,10 @@ SIGNATURES = {
     "mcgpu_batch_destroy": (i32, [vp]),
     "mcgpu_batch_options": (i32, [vp, C.POINTER(Options)]),
     "mcgpu_batch_autotune": (i32, [vp, i32, vp, C.POINTER(Options)]),
+    "mcgpu_forward_cells": (i32, [vp, i64, C.POINTER(Bank), vp, vp, vp, i32, i32, dbl, dbl, vp,
+                                  vp, vp, C.POINTER(Options), C.POINTER(Stats), vp]),
+    "mcgpu_adjoint_cells": (i32, [vp, i64, C.POINTER(Bank), vp, vp, i32, i32, dbl, dbl, vp, vp,
+                                  vp, vp, C.POINTER(Options), C.POINTER(Stats), vp]),
     "mcgpu_rng_real_step": (i32, [vp, i64, vp, vp, vp]),
     "mcgpu_rng_seed_chain": (i32, [vp, u64, i64, vp, vp]),
     "mcgpu_math_eval": (i32, [vp, C.c_int, i64, vp, vp, vp]),

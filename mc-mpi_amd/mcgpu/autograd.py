@@ -1,0 +1,76 @@
+"""torch.autograd binding of the transport: per-cell absorbed weight as a
+differentiable function of the per-cell cross sections (sig_a, sig_s).
+
+Forward: mcgpu_forward_cells on a copy of the source bank.  Backward:
+mcgpu_adjoint_cells replays every history from the saved initial bank (seed
+replay, no stored paths) with g = grad_output as target weights, giving the
+pathwise derivative for sig_a and the score-function derivative for sig_s.
+If the target weights are known when the forward runs (target_hint), the
+forward also stores each history's weighted tally and the backward replays
+each history once instead of twice.
+"""
+import torch
+
+from . import Context, make_options
+
+
+class SlabProblem:
+    """Geometry and source of one slab: the bank (dict of device tensors) is
+    the initial state of every history and is never modified."""
+
+    def __init__(self, ctx, bank, min_index, max_index, dx, particle_min_weight, options=None):
+        self.ctx = ctx
+        self.bank = bank
+        self.min_index, self.max_index = min_index, max_index
+        self.dx, self.min_w = dx, particle_min_weight
+        self.options = options if options is not None else make_options()
+
+    @property
+    def m(self):
+        return self.max_index - self.min_index
+
+    @property
+    def n(self):
+        return self.bank["x"].numel()
+
+
+class SlabTally(torch.autograd.Function):
+    """tally = SlabTally.apply(sig_a, sig_s, problem, target_hint)"""
+
+    @staticmethod
+    def forward(fctx, sig_a, sig_s, problem, target_hint=None):
+        sig_a = sig_a.detach().contiguous()
+        sig_s = sig_s.detach().contiguous()
+        work = {k: v.clone() for k, v in problem.bank.items()}
+        tally = torch.zeros(problem.m, dtype=torch.float64, device=sig_a.device)
+        score = None
+        g = None
+        if target_hint is not None:
+            g = target_hint.detach().to(torch.float64).contiguous()
+            score = torch.empty(problem.n, dtype=torch.float64, device=sig_a.device)
+        problem.ctx.forward_cells(work, sig_a, sig_s, tally, problem.min_index,
+                                  problem.max_index, problem.dx, problem.min_w, g=g,
+                                  history_score=score, options=problem.options)
+        fctx.problem = problem
+        fctx.hint = g
+        fctx.score = score
+        fctx.save_for_backward(sig_a, sig_s)
+        return tally
+
+    @staticmethod
+    def backward(fctx, grad_tally):
+        sig_a, sig_s = fctx.saved_tensors
+        p = fctx.problem
+        g = grad_tally.detach().to(torch.float64).contiguous()
+        score = None
+        if fctx.hint is not None and torch.equal(g, fctx.hint):
+            score = fctx.score
+        ga = torch.zeros_like(sig_a)
+        gs = torch.zeros_like(sig_s)
+        p.ctx.adjoint_cells(p.bank, sig_a, sig_s, p.min_index, p.max_index, p.dx, p.min_w, g,
+                            ga, gs, history_score=score, options=p.options)
+        return ga, gs, None, None
+
+
+def slab_tally(sig_a, sig_s, problem, target_hint=None):
+    return SlabTally.apply(sig_a, sig_s, problem, target_hint)

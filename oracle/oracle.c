@@ -24,6 +24,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "orc_math.h"
 
@@ -186,4 +189,184 @@ void orc64_rnd_real_step(uint64_t *seeds, int64_t n, double *out) {
 }
 void orc64_math_eval(int which, const double *x, int64_t n, double *out) {
   for (int64_t i = 0; i < n; ++i) out[i] = which == 0 ? orc64_log(x[i]) : orc64_exp(x[i]);
+}
+
+/* ============================================================ adjoint
+ * Gradient of a weighted tally T = sum_c g_c * tally_c with respect to the
+ * per-cell cross sections the step uses, sig_a_c and sig_i_c (src/layer.cpp
+ * :131-133), by replaying each history from its initial particle.
+ *
+ * For event j of a history (cell c_j, flight length d_j, weight w_j before
+ * and w_{j+1} = w_j - dw_j after), with P_j = sum_{k<j} g_{c_k} dw_k and the
+ * history total T_h = sum_k g_{c_k} dw_k:
+ *   pathwise (sig_a enters only through exp(-sig_a d)):
+ *     dT/dsig_a_c += d_j * (g_{c_j} w_{j+1} - (T_h - P_j - g_{c_j} dw_j))
+ *   score function (sig_i sets the flight-length density; the term of event
+ *   j multiplies only the tally from j on, T_h - P_j):
+ *     dT/dsig_i_c += ([collision_j] / sig_i - d_j) * (T_h - P_j)
+ *   for events with sig_i > EPS_PRECISION (otherwise the flight is not
+ *   sampled and the term is 0).
+ * The weight kill (src/layer.cpp:195) is not differentiated.
+ * Terms are summed exactly in the same fixed point as the tallies.
+ */
+
+/* one event with the cell values given directly (same arithmetic as
+   orc64_particle_step after its sig_a / sig_i products) */
+static inline double orc64_event_cells(double sig_a, double sig_i, double dx, orc64_particle *p,
+                                       double *di_out, int *coll_out) {
+  const double h = orc64_rnd_real(&p->seed);
+  double di = sig_i > 1e-4 ? -orc64_log(h) / sig_i : DBL_MAX;
+  int index_new;
+  double x_new_edge;
+  if (p->mu < 0) {
+    index_new = p->index - 1;
+    x_new_edge = p->index * dx;
+  } else {
+    index_new = p->index + 1;
+    x_new_edge = (p->index + 1) * dx;
+  }
+  double di_edge = DBL_MAX;
+  if (p->mu < -1e-4 || 1e-4 < p->mu) di_edge = (x_new_edge - p->x) / p->mu;
+  int coll = di < di_edge;
+  if (coll) {
+    index_new = p->index;
+    p->x += di * p->mu;
+    p->mu = 2 * orc64_rnd_real(&p->seed) - 1;
+  } else {
+    di = di_edge;
+    p->x = x_new_edge;
+  }
+  const double dw = (1 - orc64_exp(-sig_a * di)) * p->wmc;
+  p->wmc -= dw;
+  p->index = index_new;
+  *di_out = di;
+  *coll_out = coll;
+  return dw;
+}
+
+static inline int orc64_alive(const orc64_params *L, const orc64_particle *p) {
+  return (p->wmc >= L->min_weight) && (p->index < L->index_start + L->m) &&
+         (p->index >= L->index_start);
+}
+
+/* Forward with direct cell values; optional g -> per-history score. */
+int orc64_forward_cells(const orc64_params *L, const double *sig_a, const double *sig_i,
+                        orc64_particle *particles, int64_t n, uint64_t *fx_pairs, uint32_t *flags,
+                        const double *g, double *score, int32_t *result, int nthread) {
+  const int m = L->m;
+  orc_fx_t *fx = (orc_fx_t *)aligned_alloc(16, sizeof(orc_fx_t) * (size_t)(m > 0 ? m : 1) * (nthread + 1));
+  uint32_t *fl = (uint32_t *)calloc((size_t)m * (nthread + 1), sizeof(uint32_t));
+  if (!fx || !fl) return -1;
+  memset(fx, 0, sizeof(orc_fx_t) * (size_t)m * (nthread + 1));
+#pragma omp parallel for num_threads(nthread) schedule(dynamic, 256)
+  for (int64_t p = 0; p < n; ++p) {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    orc_fx_t *tf = fx + (size_t)t * m;
+    uint32_t *tg = fl + (size_t)t * m;
+    orc64_particle *q = &particles[p];
+    double T = 0.0;
+    while (orc64_alive(L, q)) {
+      const int c = q->index - L->index_start;
+      double di;
+      int coll;
+      const double dw = orc64_event_cells(sig_a[c], sig_i[c], L->dx, q, &di, &coll);
+      orc_fx_add(&tf[c], &tg[c], dw);
+      if (g) T = T + g[c] * dw;
+    }
+    if (score) score[p] = T;
+    if (result) {
+      int r;
+      if (q->index == L->index_start - 1) r = -1;
+      else if (q->index == L->index_start + L->m) r = 1;
+      else if (q->wmc < L->min_weight) r = 0;
+      else r = -1;
+      result[p] = r;
+    }
+  }
+  for (int t = 0; t < nthread; ++t)
+    for (int c = 0; c < m; ++c) {
+      orc_fx_t v = ((orc_fx_t)fx_pairs[2 * c + 1] << 64) | fx_pairs[2 * c];
+      v += fx[(size_t)t * m + c];
+      fx_pairs[2 * c] = (uint64_t)v;
+      fx_pairs[2 * c + 1] = (uint64_t)(v >> 64);
+      flags[c] |= fl[(size_t)t * m + c];
+    }
+  free(fx);
+  free(fl);
+  return 0;
+}
+
+/* Adjoint replay; score may be NULL (then computed by a first replay). */
+int orc64_adjoint_cells(const orc64_params *L, const double *sig_a, const double *sig_i,
+                        const orc64_particle *initial, int64_t n, const double *g,
+                        const double *score, uint64_t *ga_pairs, uint64_t *gi_pairs,
+                        uint32_t *flags, int nthread) {
+  const int m = L->m;
+  const size_t cells = (size_t)(m > 0 ? m : 1) * (nthread + 1);
+  orc_fx_t *ga = (orc_fx_t *)aligned_alloc(16, sizeof(orc_fx_t) * cells);
+  orc_fx_t *gi = (orc_fx_t *)aligned_alloc(16, sizeof(orc_fx_t) * cells);
+  uint32_t *fl = (uint32_t *)calloc(cells, sizeof(uint32_t));
+  if (!ga || !gi || !fl) return -1;
+  memset(ga, 0, sizeof(orc_fx_t) * cells);
+  memset(gi, 0, sizeof(orc_fx_t) * cells);
+#pragma omp parallel for num_threads(nthread) schedule(dynamic, 256)
+  for (int64_t p = 0; p < n; ++p) {
+    int t = 0;
+#ifdef _OPENMP
+    t = omp_get_thread_num();
+#endif
+    orc_fx_t *ta = ga + (size_t)t * m, *ti = gi + (size_t)t * m;
+    uint32_t *tg = fl + (size_t)t * m;
+    double T = 0.0;
+    if (score) {
+      T = score[p];
+    } else {
+      orc64_particle q = initial[p];
+      while (orc64_alive(L, &q)) {
+        const int c = q.index - L->index_start;
+        double di;
+        int coll;
+        const double dw = orc64_event_cells(sig_a[c], sig_i[c], L->dx, &q, &di, &coll);
+        T = T + g[c] * dw;
+      }
+    }
+    orc64_particle q = initial[p];
+    double P = 0.0;
+    while (orc64_alive(L, &q)) {
+      const int c = q.index - L->index_start;
+      double di;
+      int coll;
+      const double si = sig_i[c];
+      const double dw = orc64_event_cells(sig_a[c], si, L->dx, &q, &di, &coll);
+      const double gdw = g[c] * dw;
+      const double t_from = T - P;
+      const double a_after = t_from - gdw;
+      const double term_a = di * (g[c] * q.wmc - a_after);
+      orc_fx_add(&ta[c], &tg[c], term_a);
+      if (si > 1e-4) {
+        const double s = coll ? (1.0 / si) - di : -di;
+        orc_fx_add(&ti[c], &tg[c], s * t_from);
+      }
+      P = P + gdw;
+    }
+  }
+  for (int t = 0; t < nthread; ++t)
+    for (int c = 0; c < m; ++c) {
+      orc_fx_t v = ((orc_fx_t)ga_pairs[2 * c + 1] << 64) | ga_pairs[2 * c];
+      v += ga[(size_t)t * m + c];
+      ga_pairs[2 * c] = (uint64_t)v;
+      ga_pairs[2 * c + 1] = (uint64_t)(v >> 64);
+      v = ((orc_fx_t)gi_pairs[2 * c + 1] << 64) | gi_pairs[2 * c];
+      v += gi[(size_t)t * m + c];
+      gi_pairs[2 * c] = (uint64_t)v;
+      gi_pairs[2 * c + 1] = (uint64_t)(v >> 64);
+      flags[c] |= fl[(size_t)t * m + c];
+    }
+  free(ga);
+  free(gi);
+  free(fl);
+  return 0;
 }

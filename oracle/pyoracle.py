@@ -70,6 +70,10 @@ def _open(path):
     L.orc_fx_convert.argtypes = [vp, i64, vp, vp]
     L.orc_fx_finalize.argtypes = [vp, vp, i64, vp]
     L.orc_math_unresolved.restype = C.c_ulonglong
+    L.orc64_forward_cells.argtypes = [vp, vp, vp, vp, i64, vp, vp, vp, vp, vp, i32]
+    L.orc64_forward_cells.restype = i32
+    L.orc64_adjoint_cells.argtypes = [vp, vp, vp, vp, i64, vp, vp, vp, vp, vp, i32]
+    L.orc64_adjoint_cells.restype = i32
     return L
 
 
@@ -433,3 +437,45 @@ def math_eval(which, x, math=MATH_CR):
 
 def math_unresolved():
     return int(lib().orc_math_unresolved())
+
+
+def forward_cells(index_start, m, dx, min_w, sig_a, sig_i, bank, g=None, nthread=8):
+    """FP64 forward with per-cell (sig_a, sig_i); returns final bank, result,
+    exact tally and (if g) per-history scores."""
+    L = lib()
+    bank = np.array(bank, dtype=P64, copy=True)
+    n = len(bank)
+    sa = np.ascontiguousarray(sig_a, np.float64)
+    si = np.ascontiguousarray(sig_i, np.float64)
+    fx = np.zeros(2 * m, np.uint64)
+    fl = np.zeros(m, np.uint32)
+    gg = None if g is None else np.ascontiguousarray(g, np.float64)
+    score = None if g is None else np.zeros(n, np.float64)
+    result = np.zeros(n, np.int32)
+    prm = params(64, index_start, m, dx, min_w)
+    rc = L.orc64_forward_cells(C.addressof(prm), sa.ctypes.data, si.ctypes.data, bank.ctypes.data,
+                               n, fx.ctypes.data, fl.ctypes.data, _ptr(gg), _ptr(score),
+                               result.ctypes.data, nthread)
+    if rc:
+        raise MemoryError("oracle forward_cells failed")
+    return dict(bank=bank, result=result, tally_exact=fx_finalize(fx, fl), score=score)
+
+
+def adjoint_cells(index_start, m, dx, min_w, sig_a, sig_i, initial, g, score=None, nthread=8):
+    """FP64 adjoint (same estimator as the GPU); returns (grad_a, grad_i)."""
+    L = lib()
+    init = np.ascontiguousarray(initial, dtype=P64)
+    sa = np.ascontiguousarray(sig_a, np.float64)
+    si = np.ascontiguousarray(sig_i, np.float64)
+    gg = np.ascontiguousarray(g, np.float64)
+    sc = None if score is None else np.ascontiguousarray(score, np.float64)
+    ga = np.zeros(2 * m, np.uint64)
+    gi = np.zeros(2 * m, np.uint64)
+    fl = np.zeros(m, np.uint32)
+    prm = params(64, index_start, m, dx, min_w)
+    rc = L.orc64_adjoint_cells(C.addressof(prm), sa.ctypes.data, si.ctypes.data, init.ctypes.data,
+                               len(init), gg.ctypes.data, _ptr(sc), ga.ctypes.data,
+                               gi.ctypes.data, fl.ctypes.data, nthread)
+    if rc:
+        raise MemoryError("oracle adjoint_cells failed")
+    return fx_finalize(ga, fl), fx_finalize(gi, fl)
