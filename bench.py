@@ -1,15 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark: particle-events/s of the MI355X transport on one GPU.
 
-Workload (config 2): one slab of 10 000 cells on [0,1] with per-cell
-sigma_a ~ LogNormal(0, 0.5), sigma_s ~ LogNormal(0.5, 0.5), 1e8 particles
-entering at x = 0 with mu = sqrt(U), weight 1, minimum weight 1e-3, FP64.
-One step = one mcgpu_simulate_device call transporting the whole 1e8-particle
-bank (resident in HBM before timing) to termination, including the restore
-of the bank from a pristine copy.
+Headline (`value`): config 2 -- one slab of 10,000 cells on [0,1] with
+per-cell sigma_a ~ LogNormal(0, 0.5), sigma_s ~ LogNormal(0.5, 0.5), 1e8
+particles entering at x = 0 with mu = sqrt(U), weight 1, minimum weight 1e-3,
+FP64.  One step = one mcgpu_simulate_device call transporting the whole bank
+(resident in HBM before the timed region) to termination, including the
+restore of the bank from a pristine device copy.
 
-Prints one JSON line (see the keys below).  --gpus N > 1 is refused: this
-build runs on one GPU.
+Also reported, in "variants":
+  event     -- the event-based kernel on the same config-2 bank (one step);
+  batched   -- config 4 (4096 independent slabs, 5.8e8 particles) in one
+               persistent launch replayed from a hipGraph, launch shape chosen
+               by the autotuner on a quarter of the slabs; and the same
+               quarter of the slabs run one mcgpu_simulate_device call per
+               slab ("per-problem launches") next to its batched time;
+  adjoint   -- config 5 (1,000 cells, 1e7 particles): forward with
+               per-history scores + seed-replay adjoint, against the plain
+               forward, for d(absorbed weight in cells 800-899)/d(sigma).
+Each variant's events are device-counted.
+
+Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
 """
 import argparse
 import json
@@ -21,45 +32,237 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "mc-mpi_amd"))
 
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_EVENT = 72  # read + write of the 36-byte particle state per event
+BYTES_PER_EVENT = 72  # algorithmic: the 36-byte particle state read + written per event
+
+# HBM bytes per event measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
+# this workload (1e7 particles), scaled by the copy-kernel calibration (read
+# x2.0, write x1.0); profiles/history_config2_counters.md and
+# profiles/event_config2_hbm_counters.md hold the counter runs.
+MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0604, "event": 88.1}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def device_bank(ctx, n, torch):
-    """config 2's source bank built on the device: the reference seed chain by
-    jump-ahead (mcgpu_rng_seed_chain), mu = sqrt(U) from the same generator
-    as mcgpu.configs.config2, x = 0, index = 0, weight 1."""
+def device_bank(ctx, n, mu_fn, torch):
+    """A source bank built on the device: the reference seed chain by
+    jump-ahead (mcgpu_rng_seed_chain), x = 0, index = 0, weight 1, directions
+    from mu_fn(count, offset) (the generators of mcgpu.configs)."""
     from mcgpu import configs
     seed = torch.empty(n, dtype=torch.int64, device="cuda")
     ctx.rng_seed_chain(configs.ROOT_SEED, seed)
     mu = torch.empty(n, dtype=torch.float64, device="cuda")
-    chunk = 10_000_000
+    chunk = 20_000_000
     for o in range(0, n, chunk):
         k = min(chunk, n - o)
-        mu[o:o + k] = torch.from_numpy(configs.config2_mu(k, offset=o)).cuda()
+        mu[o:o + k] = torch.from_numpy(mu_fn(k, o)).cuda()
     return {"seed": seed, "x": torch.zeros(n, dtype=torch.float64, device="cuda"), "mu": mu,
             "wmc": torch.ones(n, dtype=torch.float64, device="cuda"),
             "index": torch.zeros(n, dtype=torch.int32, device="cuda")}
 
 
-def cpu_baseline(wl_small, seconds_hint=15.0):
-    """The FP64 oracle (port of src/layer.cpp) on a bounded prefix of the same
-    bank, all host cores up to 16, correctly rounded log/exp."""
+def restore(dst, src):
+    for k in dst:
+        dst[k].copy_(src[k])
+
+
+def cpu_baselines(n_ref, n_port):
+    """CPU legs on a prefix of the config-2 bank (n_ref resp. n_port
+    particles, each about 10 s of work), all host cores up to 16:
+      reference -- the reference's own Layer::simulate (src/layer.cpp:220-361,
+                   FP32 as the reference is built, OpenMP) from oracle/_ref;
+                   events counted by the FP32 oracle on the same bank (its
+                   per-history results equal the reference's);
+      port      -- the FP64 oracle (the same arithmetic as the GPU path).
+    Returns (reference or None when oracle/_ref is not built, port)."""
+    import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
+    from mcgpu import configs
+    wl = configs.config2(max(n_ref, n_port))
     threads = min(16, os.cpu_count() or 1)
+
     t0 = time.perf_counter()
-    o = pyoracle.simulate_bank(64, wl_small.index_start, wl_small.m, wl_small.dx,
-                               wl_small.min_weight, wl_small.sigs, wl_small.absorption_rates,
-                               wl_small.bank, nthread=threads, exact=False)
+    o = pyoracle.simulate_bank(64, wl.index_start, wl.m, wl.dx, wl.min_weight, wl.sigs,
+                               wl.absorption_rates, wl.bank[:n_port], nthread=threads,
+                               exact=False)
     dt = time.perf_counter() - t0
     ev = int(o["events"].sum())
-    return {"value": ev / dt, "unit": "particle-events/s", "cores": threads, "kind": "port",
-            "sample": f"first {len(wl_small.bank)} particles of the config-2 bank "
-                      f"({ev} events, {dt:.1f} s, FP64 oracle, correctly rounded log/exp)"}
+    port = {"value": ev / dt, "unit": "particle-events/s", "cores": threads, "kind": "port",
+            "sample": f"first {n_port} particles of the config-2 bank ({ev} events, "
+                      f"{dt:.1f} s on {threads} threads; FP64 oracle restating "
+                      "src/layer.cpp with correctly rounded log/exp)"}
+    log(f"cpu port: {port['value']:.3e} events/s")
+    if not os.path.exists(pyoracle.REF_PATH):
+        return None, port
+
+    b32 = np.zeros(n_ref, dtype=pyoracle.P32)
+    for f in ("seed", "x", "mu", "wmc", "index"):
+        b32[f] = wl.bank[f][:n_ref]
+    sigs32 = wl.sigs.astype(np.float32)
+    ar32 = wl.absorption_rates.astype(np.float32)
+    R = pyoracle.RefLayer.new(0.0, 1.0, 0, wl.m, wl.min_weight)
+    R.sigs[:] = sigs32
+    R.absorption_rates[:] = ar32
+    o32 = pyoracle.simulate_bank(32, 0, wl.m, np.float32(R.dx), np.float32(wl.min_weight), sigs32,
+                                 ar32, b32, nthread=threads)
+    ev32 = int(o32["events"].sum())
+    R.push(b32)
+    t0 = time.perf_counter()
+    R.simulate(-1, threads)
+    dt = time.perf_counter() - t0
+    assert R.nb_active == 0
+    ref = {"value": ev32 / dt, "unit": "particle-events/s", "cores": threads,
+           "kind": "reference",
+           "sample": f"first {n_ref} particles of the config-2 bank in FP32 ({ev32} events, "
+                     f"{dt:.1f} s): the reference's Layer::simulate(-1, {threads}) built from "
+                     "src/layer.cpp + src/random.cpp (-O3 -fopenmp), one rank, no MPI"}
+    log(f"cpu reference: {ref['value']:.3e} events/s")
+    return ref, port
+
+
+def run_plain(ctx, args, torch, mcgpu, configs):
+    n = args.particles
+    wl = configs.config2(n, with_bank=False)
+    log(f"config 2: building the {n}-particle bank on the device")
+    bank0 = device_bank(ctx, n, lambda k, o: configs.config2_mu(k, offset=o), torch)
+    bank = {k: v.clone() for k, v in bank0.items()}
+    sigs = torch.from_numpy(wl.sigs).cuda()
+    ar = torch.from_numpy(wl.absorption_rates).cuda()
+    w = torch.zeros(wl.m, dtype=torch.float64, device="cuda")
+
+    def step(opt):
+        restore(bank, bank0)
+        w.zero_()
+        st, _ = ctx.simulate_device(bank, sigs, ar, w, wl.min_index, wl.max_index, wl.dx,
+                                    wl.min_weight, options=opt)
+        return st
+
+    opt = mcgpu.make_options(args.kernel, block_size=args.block, particles_per_lane=args.ppl,
+                             compaction_interval=args.ci, events_per_refill=args.refill)
+    for i in range(args.warmup):
+        st = step(opt)
+        log(f"warmup {i}: {st['events']} events, kernel {st['kernel_ms']:.1f} ms")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kernel_ms, events = [], []
+    for i in range(args.steps):
+        st = step(opt)
+        kernel_ms.append(st["kernel_ms"])
+        events.append(st["events"])
+        log(f"step {i}: {st['events']} events, kernel {st['kernel_ms']:.1f} ms")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    res = {"events_per_step": events[-1], "elapsed_s": elapsed, "kernel_ms": kernel_ms,
+           "value": sum(events) / elapsed}
+    other = None
+    if not args.plain_only:
+        other_kernel = "event" if args.kernel == "history" else "history"
+        st = step(mcgpu.make_options(other_kernel))
+        torch.cuda.synchronize()
+        other = {"kernel": other_kernel, "events": st["events"], "kernel_ms": st["kernel_ms"],
+                 "events_per_s": st["events"] / st["kernel_ms"] * 1e3}
+        log(f"{other_kernel} kernel: {st['events']} events, kernel {st['kernel_ms']:.1f} ms")
+    del bank, bank0
+    torch.cuda.empty_cache()
+    return res, other
+
+
+def run_batched(ctx, args, torch, mcgpu, configs):
+    log("config 4: layout and bank")
+    slabs, sigs, ar, total = configs.config4_layout(particle_scale=args.c4_scale)
+    bank0 = device_bank(ctx, total, lambda k, o: configs.config4_mu(k, offset=o), torch)
+    bank = {k: v.clone() for k, v in bank0.items()}
+    d_sigs = torch.from_numpy(sigs).cuda()
+    d_ar = torch.from_numpy(ar).cuda()
+    w = torch.zeros(len(sigs), dtype=torch.float64, device="cuda")
+    sub = slabs[::4]
+
+    # autotune on the quarter of the slabs (inputs restored by the tuner)
+    plan = mcgpu.Batch(ctx, sub, d_sigs, d_ar, w, bank)
+    times, chosen = plan.autotune()
+    plan.close()
+    log(f"autotune (quarter): {['%.0f' % t for t in times]} ms -> {chosen}")
+    opt = mcgpu.make_options(block_size=chosen["block_size"],
+                             particles_per_lane=chosen["particles_per_lane"],
+                             events_per_refill=chosen["events_per_refill"])
+
+    # batched vs per-problem launches on the same quarter
+    restore(bank, bank0)
+    w.zero_()
+    plan = mcgpu.Batch(ctx, sub, d_sigs, d_ar, w, bank, options=opt)
+    st_sub = plan.run()
+    plan.close()
+    restore(bank, bank0)
+    w.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev_pp = 0
+    for (lo, hi, dx, mw, coff, poff, n) in sub:
+        b = {k: v[poff:poff + n] for k, v in bank.items()}
+        st, _ = ctx.simulate_device(b, d_sigs[coff:coff + hi - lo], d_ar[coff:coff + hi - lo],
+                                    w[coff:coff + hi - lo], lo, hi, dx, mw)
+        ev_pp += st["events"]
+    torch.cuda.synchronize()
+    t_pp = time.perf_counter() - t0
+    log(f"quarter: batched {st_sub['kernel_ms']:.0f} ms, per-problem {t_pp * 1e3:.0f} ms")
+
+    # the full batch
+    restore(bank, bank0)
+    w.zero_()
+    torch.cuda.synchronize()
+    plan = mcgpu.Batch(ctx, slabs, d_sigs, d_ar, w, bank, options=opt)
+    st = plan.run()
+    plan.close()
+    log(f"full batch: {st['events']} events, {st['kernel_ms']:.0f} ms")
+    out = {"workload": f"config4: {len(slabs)} slabs, {total} particles, {len(sigs)} cells",
+           "events": st["events"], "graph_ms": st["kernel_ms"],
+           "events_per_s": st["events"] / st["kernel_ms"] * 1e3, "autotuned": chosen,
+           "autotune_ms": times,
+           "quarter": {"slabs": len(sub), "events": st_sub["events"],
+                       "batched_ms": st_sub["kernel_ms"], "per_problem_ms": t_pp * 1e3,
+                       "per_problem_events": ev_pp,
+                       "batched_events_per_s": st_sub["events"] / st_sub["kernel_ms"] * 1e3,
+                       "per_problem_events_per_s": ev_pp / t_pp,
+                       "speedup": (st_sub["events"] / st_sub["kernel_ms"] * 1e3) / (ev_pp / t_pp)}}
+    del bank, bank0
+    torch.cuda.empty_cache()
+    return out
+
+
+def run_adjoint(ctx, args, torch, mcgpu, configs):
+    wl = configs.config5(args.c5_particles)
+    init = mcgpu.bank_to_device(wl.bank)
+    sa = torch.from_numpy(wl.sigma_a).cuda()
+    ss = torch.from_numpy(wl.sigma_s).cuda()
+    g = torch.from_numpy(wl.target).cuda()
+    n = len(wl.bank)
+    res = {}
+    for rep in range(2):  # first repetition warms up
+        work = {k: v.clone() for k, v in init.items()}
+        w = torch.zeros(wl.m, dtype=torch.float64, device="cuda")
+        plain = ctx.forward_cells(work, sa, ss, w, 0, wl.m, wl.dx, wl.min_weight)
+        work = {k: v.clone() for k, v in init.items()}
+        w.zero_()
+        score = torch.empty(n, dtype=torch.float64, device="cuda")
+        fwd = ctx.forward_cells(work, sa, ss, w, 0, wl.m, wl.dx, wl.min_weight, g=g,
+                                history_score=score)
+        ga = torch.zeros(wl.m, dtype=torch.float64, device="cuda")
+        gi = torch.zeros(wl.m, dtype=torch.float64, device="cuda")
+        adj = ctx.adjoint_cells(init, sa, ss, 0, wl.m, wl.dx, wl.min_weight, g, ga, gi,
+                                history_score=score)
+        adj2 = ctx.adjoint_cells(init, sa, ss, 0, wl.m, wl.dx, wl.min_weight, g, ga, gi)
+        res = {"workload": f"config5: 1000 cells, {n} particles, target cells 800-899",
+               "forward_ms": plain["kernel_ms"], "forward_events": plain["events"],
+               "forward_scored_ms": fwd["kernel_ms"], "adjoint_ms": adj["kernel_ms"],
+               "adjoint_without_scores_ms": adj2["kernel_ms"],
+               "fwd_plus_adjoint_over_forward": (fwd["kernel_ms"] + adj["kernel_ms"]) /
+               plain["kernel_ms"],
+               "events_per_s_fwd_plus_adjoint": plain["events"] /
+               (fwd["kernel_ms"] + adj["kernel_ms"]) * 1e3}
+    log(f"adjoint: {res}")
+    return res
 
 
 def main():
@@ -73,8 +276,15 @@ def main():
     ap.add_argument("--ppl", type=int, default=0, help="event kernel: pool slots per lane")
     ap.add_argument("--ci", type=int, default=0, help="event kernel: compaction interval")
     ap.add_argument("--refill", type=int, default=0, help="history kernel: events per refill")
-    ap.add_argument("--cpu-sample", type=int, default=40_000)
+    ap.add_argument("--cpu-ref-sample", type=int, default=2_000_000,
+                    help="particles for the reference CPU leg")
+    ap.add_argument("--cpu-port-sample", type=int, default=200_000,
+                    help="particles for the FP64 oracle CPU leg")
+    ap.add_argument("--c4-scale", type=float, default=1.0, help="config 4 particle scale")
+    ap.add_argument("--c5-particles", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="only the config-2 steps (for profiling runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (untimed), the "
                          "known-size reference kernel for HBM byte counters")
@@ -88,8 +298,6 @@ def main():
     import mcgpu
     from mcgpu import configs
 
-    n = args.particles
-    wl = configs.config2(n, with_bank=False)
     ctx = mcgpu.Context(0)
     if args.calibrate_copy:
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
@@ -97,65 +305,50 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    log(f"building the {n}-particle bank on the device")
-    bank0 = device_bank(ctx, n, torch)
-    bank = {k: v.clone() for k, v in bank0.items()}
-    sigs = torch.from_numpy(wl.sigs).cuda()
-    ar = torch.from_numpy(wl.absorption_rates).cuda()
-    w = torch.zeros(wl.m, dtype=torch.float64, device="cuda")
-    opt = mcgpu.make_options(args.kernel, block_size=args.block, particles_per_lane=args.ppl,
-                             compaction_interval=args.ci, events_per_refill=args.refill)
 
-    def step():
-        for k in bank:
-            bank[k].copy_(bank0[k])
-        w.zero_()
-        st, _ = ctx.simulate_device(bank, sigs, ar, w, wl.min_index, wl.max_index, wl.dx,
-                                    wl.min_weight, options=opt)
-        return st
-
-    for i in range(args.warmup):
-        st = step()
-        log(f"warmup {i}: {st['events']} events, kernel {st['kernel_ms']:.1f} ms")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    kernel_ms, events = [], []
-    for i in range(args.steps):
-        st = step()
-        kernel_ms.append(st["kernel_ms"])
-        events.append(st["events"])
-        log(f"step {i}: {st['events']} events, kernel {st['kernel_ms']:.1f} ms")
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-
-    ev_step = events[-1]
-    value = sum(events) / elapsed
-    k_avg_s = sum(kernel_ms) / len(kernel_ms) / 1e3
+    plain, other = run_plain(ctx, args, torch, mcgpu, configs)
+    ev_step = plain["events_per_step"]
+    k_avg_s = sum(plain["kernel_ms"]) / len(plain["kernel_ms"]) / 1e3
     achieved = ev_step * BYTES_PER_EVENT / k_avg_s / 1e9
+    traffic = MEASURED_TRAFFIC_PER_EVENT.get(args.kernel)
     out = {
         "metric": "particle-events/s on 1 MI355X (plain vs batched vs fwd+adjoint); % HBM roofline",
-        "value": value,
+        "value": plain["value"],
         "unit": "particle-events/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": plain["elapsed_s"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"config2: single slab, 10000 cells, {n} particles from x=0, "
-                               "mu=sqrt(U), min weight 1e-3, FP64",
+        "config": {"workload": f"config2: single slab, 10000 cells, {args.particles} particles "
+                               "from x=0, mu=sqrt(U), min weight 1e-3, FP64",
                    "kernel": args.kernel, "events_per_step": ev_step},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None if traffic is None else traffic * ev_step},
         "cpu_baseline": None,
+        "variants": {},
     }
+    if other is not None:
+        ev_o, t_o = other["events"], other["kernel_ms"] / 1e3
+        ach_o = ev_o * BYTES_PER_EVENT / t_o / 1e9
+        tr_o = MEASURED_TRAFFIC_PER_EVENT.get(other["kernel"])
+        other["roofline"] = {"bound": "hbm", "achieved": ach_o, "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": ach_o / HBM_PEAK_GBS,
+                             "traffic": None if tr_o is None else tr_o * ev_o}
+        out["variants"][other["kernel"]] = other
+    if not args.plain_only:
+        out["variants"]["batched"] = run_batched(ctx, args, torch, mcgpu, configs)
+        out["variants"]["adjoint"] = run_adjoint(ctx, args, torch, mcgpu, configs)
     if not args.no_cpu:
-        log("timing the CPU baseline")
-        small = configs.config2(args.cpu_sample)
-        out["cpu_baseline"] = cpu_baseline(small)
+        log("timing the CPU baselines")
+        ref, port = cpu_baselines(args.cpu_ref_sample, args.cpu_port_sample)
+        out["cpu_baseline"] = ref if ref is not None else port
+        out["variants"]["cpu_fp64_port"] = port
     print(json.dumps(out), flush=True)
     ctx.close()
 

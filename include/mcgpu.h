@@ -242,6 +242,19 @@ int mcgpu_rng_real_step(mcgpu_context *ctx, int64_t n, uint64_t *seeds, double *
 int mcgpu_rng_seed_chain(mcgpu_context *ctx, uint64_t seed, int64_t n, uint64_t *out,
                          void *stream);
 
+/* ------------------------------------------------------------- sources
+ * Device-side Layer::create_particles (src/layer.cpp:89-121): writes
+ * particles first .. first+n-1 of the source that create_particles(x_ini,
+ * wmc, ..., seed) (src/layer.cpp:71-82) would push, into bank[0..n) (device
+ * SoA).  Particle i gets seed = the (first+i+1)-th rnd_seed output from
+ * `seed`, then mu = 2 * rnd_real(&seed) - 1, x = x_ini, wmc = wmc and
+ * index = (int)(x_ini / dx).  Chain positions are reached by jump-ahead, so
+ * lazy creation in chunks passes first = particles already created.
+ */
+int mcgpu_create_particles(mcgpu_context *ctx, uint64_t seed, int64_t first, int64_t n,
+                           double x_ini, double wmc, double dx, const mcgpu_bank *bank,
+                           void *stream);
+
 /* Evaluate the device log (which = 0) or exp (which = 1) used by the
    transport kernels on n inputs (device pointers). */
 int mcgpu_math_eval(mcgpu_context *ctx, int which, int64_t n, const double *x, double *out,

@@ -88,14 +88,11 @@ __global__ void k_rng_real_step(int64_t n, uint64_t *seeds, double *reals) {
   }
 }
 
-/* out[i] = (i+1)-th output of rnd_seed from `seed` (src/random.cpp:24-29),
-   via the affine map f(s) = G s + C raised to the power i+1 by squaring; the
+/* The k-th output of rnd_seed from `seed` (src/random.cpp:24-29), k >= 1,
+   via the affine map f(s) = G s + C raised to the power k by squaring; the
    map is taken mod 2^64 and reduced mod 2^63 at the end, which is exact
    because 2^63 divides 2^64. */
-__global__ void k_seed_chain(uint64_t seed, int64_t n, uint64_t *out) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint64_t k = (uint64_t)i + 1;
+__device__ __forceinline__ uint64_t seed_chain_at(uint64_t seed, uint64_t k) {
   uint64_t A = MC_RNGS_G, B = MC_RNGS_C; /* f^(2^j) */
   uint64_t RA = 1, RB = 0;               /* accumulated power */
   while (k) {
@@ -107,7 +104,30 @@ __global__ void k_seed_chain(uint64_t seed, int64_t n, uint64_t *out) {
     A = A * A;
     k >>= 1;
   }
-  out[i] = (RA * seed + RB) & MC_MASK63;
+  return (RA * seed + RB) & MC_MASK63;
+}
+
+/* out[i] = (i+1)-th output of rnd_seed from `seed`. */
+__global__ void k_seed_chain(uint64_t seed, int64_t n, uint64_t *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = seed_chain_at(seed, (uint64_t)i + 1);
+}
+
+/* Layer::create_particles(int) (src/layer.cpp:89-121) for particles
+   first .. first+n-1 of a source: particle i takes the (first+i+1)-th chain
+   output as its seed, then draws mu = 2 rnd_real - 1 from it. */
+__global__ void k_create_particles(uint64_t seed, int64_t first, int64_t n, double x_ini,
+                                   double wmc, int32_t index, mcgpu_bank b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t s = seed_chain_at(seed, (uint64_t)(first + i) + 1);
+  const double mu = 2 * mc_rnd_real(s) - 1;
+  b.seed[i] = s;
+  b.x[i] = x_ini;
+  b.mu[i] = mu;
+  b.wmc[i] = wmc;
+  b.index[i] = index;
 }
 
 __global__ void k_math_eval(int which, int64_t n, const double *x, double *out) {
@@ -367,6 +387,25 @@ int mcgpu_rng_seed_chain(mcgpu_context *ctx, uint64_t seed, int64_t n, uint64_t 
   MC_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = pick(ctx, stream);
   hipLaunchKernelGGL(k_seed_chain, dim3(grid_for(n, 256)), dim3(256), 0, s, seed, n, out);
+  MC_HIP(ctx, hipGetLastError());
+  MC_HIP(ctx, hipStreamSynchronize(s));
+  return MCGPU_OK;
+}
+
+int mcgpu_create_particles(mcgpu_context *ctx, uint64_t seed, int64_t first, int64_t n,
+                           double x_ini, double wmc, double dx, const mcgpu_bank *bank,
+                           void *stream) {
+  if (!ctx || n < 0 || first < 0) return MCGPU_ERR_ARG;
+  if (n == 0) return MCGPU_OK;
+  if (!bank || !bank->seed || !bank->x || !bank->mu || !bank->wmc || !bank->index)
+    return mc_fail(ctx, MCGPU_ERR_ARG, "bank has a null array");
+  const double cell = x_ini / dx;
+  if (!(dx > 0) || !(cell >= 0) || !(cell < 2147483647.0))
+    return mc_fail(ctx, MCGPU_ERR_ARG, "x_ini / dx is not a cell index");
+  MC_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = pick(ctx, stream);
+  hipLaunchKernelGGL(k_create_particles, dim3(grid_for(n, 256)), dim3(256), 0, s, seed, first,
+                     n, x_ini, wmc, (int32_t)cell, *bank);
   MC_HIP(ctx, hipGetLastError());
   MC_HIP(ctx, hipStreamSynchronize(s));
   return MCGPU_OK;

@@ -200,6 +200,14 @@ class Context:
         self.check(lib().mcgpu_rng_seed_chain(self._h, seed, out.numel(), _ptr(out),
                                               _stream(stream)))
 
+    def create_particles(self, seed, first, bank, x_ini, wmc, dx, stream=None):
+        """mcgpu_create_particles: fill the device bank (dict of tensors, n =
+        its length) with source particles first .. first+n-1."""
+        b = Bank(_ptr(bank["seed"]), _ptr(bank["x"]), _ptr(bank["mu"]), _ptr(bank["wmc"]),
+                 _ptr(bank["index"]))
+        self.check(lib().mcgpu_create_particles(self._h, seed, first, bank["x"].numel(), x_ini,
+                                                wmc, dx, C.byref(b), _stream(stream)))
+
     def math_eval(self, which, x, out, stream=None):
         w = {"log": 0, "exp": 1}[which]
         self.check(lib().mcgpu_math_eval(self._h, w, x.numel(), _ptr(x), _ptr(out),

@@ -89,6 +89,8 @@ SIGNATURES = {
     "mcgpu_rng_real_step": (i32, [vp, i64, vp, vp, vp]),
     "mcgpu_rng_seed_chain": (i32, [vp, u64, i64, vp, vp]),
     "mcgpu_math_eval": (i32, [vp, C.c_int, i64, vp, vp, vp]),
+    "mcgpu_create_particles": (i32, [vp, u64, i64, i64, C.c_double, C.c_double, C.c_double, vp,
+                                     vp]),
 }
 
 

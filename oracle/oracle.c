@@ -344,10 +344,22 @@ int orc64_adjoint_cells(const orc64_params *L, const double *sig_a, const double
       const double gdw = g[c] * dw;
       const double t_from = T - P;
       const double a_after = t_from - gdw;
+#if ORC_MUTANT == 6 /* planted mistake: pathwise term without the downstream tally */
+      const double term_a = di * (g[c] * q.wmc);
+      (void)a_after;
+#else
       const double term_a = di * (g[c] * q.wmc - a_after);
+#endif
       orc_fx_add(&ta[c], &tg[c], term_a);
       if (si > 1e-4) {
+#if ORC_MUTANT == 7 /* planted mistake: sign of the flight-length score */
+        const double s = coll ? (1.0 / si) + di : di;
+#elif ORC_MUTANT == 8 /* planted mistake: collision density term dropped */
+        const double s = -di;
+        (void)coll;
+#else
         const double s = coll ? (1.0 / si) - di : -di;
+#endif
         orc_fx_add(&ti[c], &tg[c], s * t_from);
       }
       P = P + gdw;

@@ -92,6 +92,38 @@ def test_device_rng_known_answers(ctx, oracle):
     assert np.array_equal(reals.cpu().numpy().view(np.uint64), r.view(np.uint64))
 
 
+def test_device_create_particles(ctx, oracle):
+    """mcgpu_create_particles against the oracle's Layer: decompose_domain with
+    the reference's config.yaml source (x_ini = 0.7071..., 1000 cells, seed
+    root 5127801, weight 1/N) creates its first MAX_PARTICLES_VECTOR = 1e6
+    particles eagerly (src/layer.cpp:71-82); a later lazy chunk (particles
+    1e6 .. 1.3e6) against the oracle's seed chain + one rnd_real step."""
+    import torch
+    import mcgpu
+    x_ini, n_total = 0.7071067690849304, 2_500_000
+    L = oracle.OracleLayer.decompose_domain(64, 0.0, 1.0, x_ini, 1, 0, 1000, n_total, 1e-12)
+    want = L.particles.copy()
+    assert len(want) == 1_000_000
+
+    def dev_bank(n):
+        return {"seed": torch.empty(n, dtype=torch.int64, device="cuda"),
+                "x": torch.empty(n, dtype=torch.float64, device="cuda"),
+                "mu": torch.empty(n, dtype=torch.float64, device="cuda"),
+                "wmc": torch.empty(n, dtype=torch.float64, device="cuda"),
+                "index": torch.empty(n, dtype=torch.int32, device="cuda")}
+    b = dev_bank(len(want))
+    ctx.create_particles(5127801, 0, b, x_ini, 1.0 / n_total, L.dx)
+    _assert_bank_equal(mcgpu.bank_from_device(b), want)
+    first, n = 1_000_000, 300_000
+    b = dev_bank(n)
+    ctx.create_particles(5127801, first, b, x_ini, 1.0 / n_total, L.dx)
+    got = mcgpu.bank_from_device(b)
+    seeds, r = oracle.rnd_real_step(oracle.rnd_seed_chain(5127801, first + n)[first:])
+    assert np.array_equal(got["seed"], seeds)
+    assert np.array_equal(got["mu"].view(np.uint64), (2 * r - 1).view(np.uint64))
+    assert np.all(got["index"] == want["index"][0]) and np.all(got["x"] == x_ini)
+
+
 # ------------------------------------------------------------- transport
 
 CASES = [("config1", 20_000), ("config3", 20_000), ("config5", 3_000), ("config2", 400)]
