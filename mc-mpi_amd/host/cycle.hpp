@@ -1,0 +1,63 @@
+/*
+ * cycle.hpp -- mcgpu::SyncCycle: mc-mpi's synchronous worker
+ * (src/worker_sync.cpp:24-135) with every rank of a domain decomposition held
+ * in one process and transported on one GPU.
+ *
+ * The reference runs one MPI rank per layer.  Each cycle every rank calls
+ * Layer::simulate(nb_particles_per_cycle); then escape buffers go to the
+ * neighbours in two phases (odd ranks with their right neighbour, then even
+ * ranks with theirs, src/worker_sync.cpp:55-109), each received buffer being
+ * appended to the receiver's particles in sender order; the loop ends when
+ * the ranks' nb_disabled add up to nb_particles (src/worker_sync.cpp:111-119).
+ * Here the layers are mcgpu::Layer objects (GPU transport) and the exchange
+ * moves vectors in host memory, in the same order, so the run reproduces
+ * `mpirun -n world_size ./main config.yaml sync` without MPI.
+ *
+ * dump() writes the reference's output files (src/worker.cpp:37-61):
+ * folder/config.yaml (src/worker.cpp:218-241), folder/weights.csv
+ * (src/worker.cpp:287-313), WA.out of rank 0's layer (src/layer.cpp:363-380)
+ * and folder/stats.csv (src/worker.cpp:63-181; one row per rank covering the
+ * whole run, with time_idle 0).
+ */
+#ifndef MCGPU_CYCLE_HPP
+#define MCGPU_CYCLE_HPP
+
+#include <string>
+#include <vector>
+
+#include "layer.hpp"
+#include "yaml_loader.hpp"
+
+namespace mcgpu {
+
+class SyncCycle {
+public:
+  /* decompose_domain(..., options.world_size, r, ...) for every rank r */
+  explicit SyncCycle(const MCMPIOptions &options);
+
+  /* WorkerSync::spin; returns the number of cycles.  Throws mcgpu::Error if
+     max_cycles (> 0) pass without every particle being disabled. */
+  long long spin(long long max_cycles = -1);
+
+  /* Worker::gather_weights_absorbed (src/worker.cpp:183-216): the ranks'
+     tallies concatenated in rank order; displs receives each rank's offset. */
+  std::vector<real_t> gather_weights_absorbed(std::vector<int> *displs = nullptr) const;
+
+  /* Worker::dump (src/worker.cpp:37-61) */
+  void dump(const std::string &folder, const std::string &wa_path = "WA.out") const;
+
+  MCMPIOptions options;
+  std::vector<Layer> layers;
+  long long nb_cycles = 0;
+  unsigned long long events = 0; /* device-counted over all ranks and cycles */
+
+  struct RankTimes {
+    double comp = 0, send = 0, recv = 0; /* seconds: transport, exchange, termination sum */
+  };
+  std::vector<RankTimes> times;
+  double elapsed = 0; /* seconds of the last spin() */
+};
+
+} // namespace mcgpu
+
+#endif

@@ -74,6 +74,7 @@ Layer::Layer(real_t x_min, real_t x_max, int index_start, int m, real_t particle
       particle_min_weight(particle_min_weight) {
   mcgpu_default_options(&options);
   last_stats = mcgpu_stats{};
+  total_stats = mcgpu_stats{};
   sigs.reserve(m);
   real_t x_mid;
   for (int i = 0; i < m; ++i) {
@@ -143,7 +144,10 @@ void Layer::simulate(int nb_particles, int nthread, bool use_gpu) {
     create_particles(nb_particles);
 
   nb_particles = std::min((int)particles.size(), nb_particles);
-  if (nb_particles <= 0) return;
+  if (nb_particles <= 0) {
+    last_stats = mcgpu_stats{};
+    return;
+  }
 
   const int particles_size = (int)particles.size();
   Particle *bank = particles.data() + (particles_size - nb_particles);
@@ -155,6 +159,12 @@ void Layer::simulate(int nb_particles, int nthread, bool use_gpu) {
   if (rc != MCGPU_OK)
     throw Error(rc, std::string("mcgpu_simulate: ") + mcgpu_status_string(rc) + ": " +
                         mcgpu_context_error(default_context()));
+  total_stats.events += last_stats.events;
+  total_stats.collisions += last_stats.collisions;
+  total_stats.n_left += last_stats.n_left;
+  total_stats.n_right += last_stats.n_right;
+  total_stats.n_killed += last_stats.n_killed;
+  total_stats.kernel_ms += last_stats.kernel_ms;
 
   /* iteration i transported particles[size-1-i] (src/layer.cpp:332-346) */
   for (int i = 0; i < nb_particles; i++) {

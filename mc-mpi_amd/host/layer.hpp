@@ -58,8 +58,11 @@ public:
 
   /* options of the GPU transport (kernel choice, launch shape) */
   mcgpu_options options;
-  /* device-counted totals of the last simulate call */
+  /* device-counted totals of the last transport (mcgpu_simulate call) made
+     by simulate(); zero when the last simulate() transported nothing */
   mcgpu_stats last_stats;
+  /* the same, summed over every transport of this Layer */
+  mcgpu_stats total_stats;
 
   // -- Data --
   const real_t x_min, x_max;

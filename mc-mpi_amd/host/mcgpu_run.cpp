@@ -1,53 +1,57 @@
 /*
- * mcgpu_run -- single-GPU run of a config.yaml, the one-rank case of
- * `mpirun -n 1 ./main config.yaml sync` (src/main.cpp:51-96,
- * src/worker_sync.cpp:24-135): decompose_domain for rank 0 of 1, then
- * Layer::simulate(nb_particles_per_cycle) cycles until every particle is
- * disabled; writes WA.out (src/layer.cpp:363-380) and weights.csv
- * (src/worker.cpp:287-313) and prints the elapsed seconds.
+ * mcgpu_run -- the run of `mpirun -n R ./main config.yaml sync`
+ * (src/main.cpp:51-96, src/worker_sync.cpp:24-135) with all R layers on one
+ * GPU: decompose_domain for every rank, synchronous cycles until every
+ * particle is disabled, then the reference's outputs (src/worker.cpp:37-61):
+ * out/config.yaml, out/weights.csv, out/stats.csv and WA.out of rank 0.
+ * Prints the elapsed seconds on stdout, like src/main.cpp:90-91.
  *
- *   mcgpu_run config.yaml [history|event] [device]
+ *   mcgpu_run config.yaml [history|event] [device] [--ranks R] [--out DIR]
  */
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
-#include <chrono>
+#include <string>
 
-#include "layer.hpp"
-#include "yaml_loader.hpp"
+#include "cycle.hpp"
 
 int main(int argc, char **argv) {
   if (argc < 2) {
-    fprintf(stderr, "usage: %s config.yaml [history|event] [device]\n", argv[0]);
+    fprintf(stderr, "usage: %s config.yaml [history|event] [device] [--ranks R] [--out DIR]\n",
+            argv[0]);
+    return 2;
+  }
+  int ranks = 1, kernel = MCGPU_KERNEL_HISTORY, device = -1, positional = 0;
+  std::string out = "out";
+  for (int i = 2; i < argc; ++i) {
+    if (!strcmp(argv[i], "--ranks") && i + 1 < argc) {
+      ranks = atoi(argv[++i]);
+    } else if (!strcmp(argv[i], "--out") && i + 1 < argc) {
+      out = argv[++i];
+    } else if (positional == 0) {
+      kernel = strcmp(argv[i], "event") == 0 ? MCGPU_KERNEL_EVENT : MCGPU_KERNEL_HISTORY;
+      ++positional;
+    } else {
+      device = atoi(argv[i]);
+    }
+  }
+  if (ranks < 1) {
+    fprintf(stderr, "mcgpu_run: --ranks must be >= 1\n");
     return 2;
   }
   try {
-    const mcgpu::MCMPIOptions opt = mcgpu::options_from_config(argv[1], 1);
-    if (argc > 3) mcgpu::set_device(atoi(argv[3]));
-    auto t0 = std::chrono::steady_clock::now();
-    mcgpu::Layer layer = mcgpu::decompose_domain(opt.x_min, opt.x_max, opt.x_ini, 1, 0,
-                                                 opt.nb_cells, (int)opt.nb_particles,
-                                                 opt.particle_min_weight);
-    if (argc > 2 && strcmp(argv[2], "event") == 0) layer.options.kernel = MCGPU_KERNEL_EVENT;
-    unsigned long long events = 0;
-    while (layer.nb_disabled < opt.nb_particles) {
-      layer.simulate(opt.nb_particles_per_cycle, opt.nthread, true);
-      events += layer.last_stats.events;
-      if (layer.nb_active() == 0 && layer.particles_left.empty() && layer.particles_right.empty())
-        break;
-    }
-    const double secs =
-        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    layer.dump_WA();
-    FILE *f = fopen("weights.csv", "w");
-    if (f) {
-      fprintf(f, "proc, x, weight\n");
-      for (size_t i = 0; i < layer.weights_absorbed.size(); ++i)
-        fprintf(f, "%d, %.18e, %.18e\n", 0, layer.dx * (i + 0.5), layer.weights_absorbed[i] / layer.dx);
-      fclose(f);
-    }
-    printf("%lf\n", secs);
-    fprintf(stderr, "events %llu, disabled %d\n", events, layer.nb_disabled);
+    if (device >= 0) mcgpu::set_device(device);
+    const mcgpu::MCMPIOptions opt = mcgpu::options_from_config(argv[1], ranks);
+    mcgpu::SyncCycle run(opt);
+    for (mcgpu::Layer &L : run.layers) L.options.kernel = kernel;
+    run.spin();
+    printf("%lf\n", run.elapsed);
+    run.dump(out);
+    long long disabled = 0;
+    for (const mcgpu::Layer &L : run.layers) disabled += L.nb_disabled;
+    fprintf(stderr, "events %llu, disabled %lld, cycles %lld, ranks %d\n", run.events, disabled,
+            run.nb_cycles, ranks);
   } catch (const std::exception &e) {
     fprintf(stderr, "mcgpu_run: %s\n", e.what());
     return 1;

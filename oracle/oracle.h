@@ -23,6 +23,9 @@ int64_t orc64_layer_array(orc64_Layer *L, int which, void **ptr);
 int64_t orc64_layer_int(const orc64_Layer *L, int which);
 void orc64_layer_set_tally_exact(orc64_Layer *L, int on);
 int orc64_layer_push(orc64_Layer *L, const void *p, int64_t n);
+/* src/worker_sync.cpp:24-135 over layers[0..world_size) (see orc_impl.inc) */
+int64_t orc64_sync_run(orc64_Layer **layers, int world_size, int nb_particles,
+                       int nb_particles_per_cycle, int nthread, int64_t max_cycles);
 void orc_set_math_mode(int mode);
 #ifdef __cplusplus
 }

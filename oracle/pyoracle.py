@@ -58,6 +58,9 @@ def _open(path):
         getattr(L, pfx + "layer_push").argtypes = [vp, vp, i64]
         getattr(L, pfx + "layer_push").restype = i32
         getattr(L, pfx + "simulate_bank").restype = i32
+        getattr(L, pfx + "layer_tally_fx").argtypes = [vp, vp, vp]
+        getattr(L, pfx + "sync_run").argtypes = [vp, i32, i32, i32, i32, i64]
+        getattr(L, pfx + "sync_run").restype = i64
     L.orc64_layer_set_tally_exact.argtypes = [vp, i32]
     L.orc64_simulate_bank_fx.argtypes = [vp, vp, vp, vp, i64, vp, vp, vp, vp, vp, vp, i32]
     L.orc64_simulate_bank_fx.restype = i32
@@ -258,6 +261,15 @@ class OracleLayer(_LayerBase):
         assert self.precision == 64
         lib().orc64_layer_set_tally_exact(self.h, 1 if on else 0)
 
+    def tally_exact_total(self):
+        """Exact mode: the tally summed exactly over all simulate calls,
+        rounded once to double."""
+        m = int(self._int_fn(self.h, 5))
+        pairs = np.zeros(2 * m, np.uint64)
+        flags = np.zeros(m, np.uint32)
+        lib().orc64_layer_tally_fx(self.h, pairs.ctypes.data, flags.ctypes.data)
+        return fx_finalize(pairs, flags)
+
     @property
     def dx(self):
         return getattr(lib(), self.pfx + "layer_real")(self.h, 2)
@@ -277,6 +289,25 @@ class OracleLayer(_LayerBase):
     @property
     def collisions(self):
         return int(self._int_fn(self.h, 8))
+
+
+def sync_run(precision, x_min, x_max, x_ini, world_size, nb_cells, nb_particles, min_w,
+             nb_particles_per_cycle, nthread=1, tally_exact=False, max_cycles=10_000_000):
+    """The synchronous worker loop (src/worker_sync.cpp:24-135) for all ranks
+    of one decomposition in this process.  Returns (layers, cycles)."""
+    layers = [OracleLayer.decompose_domain(precision, x_min, x_max, x_ini, world_size, r,
+                                           nb_cells, nb_particles, min_w)
+              for r in range(world_size)]
+    if tally_exact:
+        for L in layers:
+            L.set_tally_exact(True)
+    arr = (C.c_void_p * world_size)(*[L.h for L in layers])
+    pfx = "orc32_" if precision == 32 else "orc64_"
+    cycles = getattr(lib(), pfx + "sync_run")(C.cast(arr, C.c_void_p), world_size, nb_particles,
+                                             nb_particles_per_cycle, nthread, max_cycles)
+    if cycles < 0:
+        raise RuntimeError(f"oracle sync_run failed ({cycles})")
+    return layers, cycles
 
 
 class RefLayer(_LayerBase):

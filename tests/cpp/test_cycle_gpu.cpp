@@ -1,0 +1,116 @@
+// test_cycle_gpu.cpp -- mcgpu::SyncCycle (every rank of a decomposition on
+// one GPU) against the oracle's restatement of the synchronous worker loop
+// (orc64_sync_run, src/worker_sync.cpp:24-135).
+//
+// For each case both run the whole loop; then the number of cycles, the
+// device-counted events, and for every rank the particles, escape buffers
+// (bytes), nb_disabled and weights_absorbed (bits; both add each call's
+// exact tally) must be identical.
+#include <stdio.h>
+#include <string.h>
+
+#include <vector>
+
+#include "cycle.hpp"
+#include "oracle.h"
+
+static int failures = 0;
+#define CHECK(cond, ...)                                                   \
+  do {                                                                     \
+    if (!(cond)) {                                                         \
+      fprintf(stderr, "FAIL %s:%d: ", __FILE__, __LINE__);                 \
+      fprintf(stderr, __VA_ARGS__);                                        \
+      fprintf(stderr, "\n");                                               \
+      ++failures;                                                          \
+    }                                                                      \
+  } while (0)
+
+template <class T>
+static bool same_bytes(const std::vector<T> &a, const void *b, int64_t nb) {
+  if ((int64_t)a.size() != nb) return false;
+  return nb == 0 || memcmp(a.data(), b, sizeof(T) * nb) == 0;
+}
+
+struct Case {
+  const char *name;
+  int world_size, nb_cells, nb_particles, per_cycle;
+  double x_ini, min_w;
+  int kernel;
+};
+
+static void run(const Case &c) {
+  mcgpu::MCMPIOptions opt{};
+  opt.world_size = c.world_size;
+  opt.nb_cells = c.nb_cells;
+  opt.x_min = 0.0;
+  opt.x_max = 1.0;
+  opt.x_ini = c.x_ini;
+  opt.particle_min_weight = c.min_w;
+  opt.nb_particles = c.nb_particles;
+  opt.cycle_time = 1e-5;
+  opt.nb_particles_per_cycle = c.per_cycle;
+  opt.nthread = 1;
+  opt.statistics_cycle_time = 0.1;
+  mcgpu::SyncCycle g(opt);
+  for (auto &L : g.layers) L.options.kernel = c.kernel;
+  g.spin(10000000);
+
+  std::vector<orc64_Layer *> O(c.world_size);
+  for (int r = 0; r < c.world_size; ++r) {
+    O[r] = orc64_decompose_domain(0.0, 1.0, c.x_ini, c.world_size, r, c.nb_cells, c.nb_particles,
+                                  c.min_w);
+    orc64_layer_set_tally_exact(O[r], 1);
+  }
+  const int64_t cycles = orc64_sync_run(O.data(), c.world_size, c.nb_particles, c.per_cycle, 8,
+                                        10000000);
+  CHECK(cycles == g.nb_cycles, "%s cycles %lld vs %lld", c.name, (long long)g.nb_cycles,
+        (long long)cycles);
+  unsigned long long events = 0;
+  long long disabled = 0;
+  for (int r = 0; r < c.world_size; ++r) {
+    mcgpu::Layer &G = g.layers[r];
+    void *p;
+    int64_t n;
+    n = orc64_layer_array(O[r], 3, &p);
+    CHECK(same_bytes(G.particles, p, n), "%s rank %d particles", c.name, r);
+    n = orc64_layer_array(O[r], 4, &p);
+    CHECK(same_bytes(G.particles_left, p, n), "%s rank %d particles_left", c.name, r);
+    n = orc64_layer_array(O[r], 5, &p);
+    CHECK(same_bytes(G.particles_right, p, n), "%s rank %d particles_right", c.name, r);
+    n = orc64_layer_array(O[r], 2, &p);
+    CHECK(same_bytes(G.weights_absorbed, p, n), "%s rank %d weights_absorbed", c.name, r);
+    CHECK(G.nb_disabled == orc64_layer_int(O[r], 0), "%s rank %d nb_disabled %d vs %lld", c.name,
+          r, G.nb_disabled, (long long)orc64_layer_int(O[r], 0));
+    events += (unsigned long long)orc64_layer_int(O[r], 7);
+    disabled += G.nb_disabled;
+    orc64_layer_free(O[r]);
+  }
+  CHECK(events == g.events, "%s events %llu vs %llu", c.name, g.events, events);
+  CHECK(disabled == c.nb_particles, "%s disabled %lld", c.name, disabled);
+  printf("ok %s: %lld cycles, %llu events\n", c.name, g.nb_cycles, g.events);
+  fflush(stdout);
+}
+
+int main() {
+  const double x_ini = 7.071067690849304199e-01, min_w = 9.999999960041972003e-13;
+  const std::vector<Case> cases = {
+      {"config.yaml source, 1 rank", 1, 1000, 20000, 500, x_ini, min_w, MCGPU_KERNEL_HISTORY},
+      {"config.yaml source, 2 ranks", 2, 1000, 20000, 500, x_ini, min_w, MCGPU_KERNEL_HISTORY},
+      {"config.yaml source, 5 ranks", 5, 1000, 20000, 500, x_ini, min_w, MCGPU_KERNEL_HISTORY},
+      {"uneven split, 7 ranks, source in rank 0", 7, 1000, 10000, 300, 0.05, 1e-6,
+       MCGPU_KERNEL_HISTORY},
+      {"event kernel, 4 ranks", 4, 1024, 10000, 700, 0.3, 1e-6, MCGPU_KERNEL_EVENT},
+  };
+  try {
+    for (const Case &c : cases) run(c);
+  } catch (const std::exception &e) {
+    fprintf(stderr, "exception: %s\n", e.what());
+    return 1;
+  }
+  if (failures) {
+    fprintf(stderr, "%d failures\n", failures);
+    return 1;
+  }
+  printf("all ok\n");
+  return 0;
+}

@@ -35,9 +35,10 @@ def test_layer_against_oracle(tmp_path):
 
 @pytest.mark.gpu
 def test_mcgpu_run_cli_on_reference_config(tmp_path):
-    """The reference's config.yaml (1000 cells, 1e5 particles, default law)
-    through mcgpu_run: every particle is disabled, WA.out and weights.csv are
-    written, and the absorbed fraction matches the oracle's FP64 Layer."""
+    """The reference's config.yaml source (1000 cells, default law; 2e4
+    particles) through mcgpu_run: every particle is disabled, WA.out and
+    out/{weights.csv,config.yaml,stats.csv} are written, and the weights
+    equal the oracle's FP64 Layer run through the same cycles."""
     import numpy as np
     cfg = tmp_path / "config.yaml"
     cfg.write_text("x_min: 0.000000000000000000e+00\nx_max: 1.000000000000000000e+00\n"
@@ -48,7 +49,8 @@ def test_mcgpu_run_cli_on_reference_config(tmp_path):
     assert out.returncode == 0
     assert "disabled 20000" in out.stderr
     wa = np.loadtxt(tmp_path / "WA.out")
-    w = np.loadtxt(tmp_path / "weights.csv", delimiter=",", skiprows=1)
+    w = np.loadtxt(tmp_path / "out" / "weights.csv", delimiter=",", skiprows=1)
+    assert (tmp_path / "out" / "config.yaml").exists() and (tmp_path / "out" / "stats.csv").exists()
     assert wa.shape == (1000, 2) and w.shape == (1000, 3)
     import pyoracle
     L = pyoracle.OracleLayer.decompose_domain(64, 0.0, 1.0, 7.071067690849304199e-01, 1, 0, 1000,
@@ -57,3 +59,41 @@ def test_mcgpu_run_cli_on_reference_config(tmp_path):
     while L.nb_disabled < 20000:
         L.simulate(5000, 8)
     np.testing.assert_array_equal(w[:, 2], L.weights_absorbed / L.dx)
+
+
+@pytest.mark.gpu
+def test_sync_cycle_against_oracle(tmp_path):
+    """mcgpu::SyncCycle (every rank of a decomposition on one GPU) vs the
+    oracle's synchronous worker loop: cycles, events, and per rank the
+    particles, escape buffers, nb_disabled and weights, bit for bit."""
+    exe = os.path.join(BUILD, "test_cycle_gpu")
+    assert os.path.exists(exe)
+    out = _run([exe], str(tmp_path))
+    assert out.returncode == 0 and "all ok" in out.stdout
+
+
+@pytest.mark.gpu
+def test_mcgpu_run_five_ranks_on_one_gpu(tmp_path):
+    """`mcgpu_run config.yaml --ranks 5` stands in for `mpirun -n 5 ./main
+    config.yaml sync`: out/weights.csv holds the five ranks' tallies in rank
+    order (proc column 0..4) and equals the oracle's five-rank loop."""
+    import numpy as np
+    import pyoracle
+    cfg = tmp_path / "config.yaml"
+    cfg.write_text("x_min: 0.000000000000000000e+00\nx_max: 1.000000000000000000e+00\n"
+                   "x_ini: 7.071067690849304199e-01\nparticle_min_weight: 9.999999960041972003e-13\n"
+                   "nb_cells: 1000\nnb_particles: 20000\ncycle_time: 1.e-05\n"
+                   "nb_particles_per_cycle: 500\nnthread: 1\nstatistics_cycle_time: 1.e-1\n")
+    out = _run([os.path.join(ROOT, "mc-mpi_amd", "mcgpu_run"), str(cfg), "--ranks", "5"],
+               str(tmp_path))
+    assert out.returncode == 0 and "disabled 20000" in out.stderr and "ranks 5" in out.stderr
+    w = np.loadtxt(tmp_path / "out" / "weights.csv", delimiter=",", skiprows=1)
+    assert w.shape == (1000, 3)
+    assert np.array_equal(w[:, 0], np.repeat(np.arange(5), 200))
+    layers, cycles = pyoracle.sync_run(64, 0.0, 1.0, 7.071067690849304199e-01, 5, 1000, 20000,
+                                       9.999999960041972003e-13, 500, nthread=8, tally_exact=True)
+    want = np.concatenate([L.weights_absorbed for L in layers]) / layers[0].dx
+    np.testing.assert_array_equal(w[:, 2], want)
+    stats = (tmp_path / "out" / "stats.csv").read_text().splitlines()
+    assert stats[0].startswith("rank, starttime, endtime, time_comp") and len(stats) == 6
+    assert all(line.endswith(f"{cycles}, ") for line in stats[1:])

@@ -387,3 +387,54 @@ def test_pins_catch_planted_mistakes(oracle, mutant, tmp_path):
     assert not all(results.values()), results
     # and the faithful build passes all of them
     assert _golden_equal(oracle) and _absorber_closed_form_holds(oracle)
+
+
+# ------------------------------------------------------ synchronous cycle loop
+
+def _sync_totals(po_mod, R, n=20_000, max_cycles=10_000_000):
+    layers, cycles = po_mod.sync_run(64, 0.0, 1.0, 0.7071067690849304, R, 1024, n, 1e-6, 500,
+                                     nthread=4, tally_exact=True, max_cycles=max_cycles)
+    w = np.concatenate([L.tally_exact_total() for L in layers])
+    return w, sum(L.nb_disabled for L in layers), sum(L.events for L in layers), cycles
+
+
+def test_sync_cycle_loop_reproduces_the_single_layer(oracle):
+    """src/worker_sync.cpp:24-135 restated for R ranks in one process.  With
+    1024 cells on [0,1] every layer's dx is exactly 2^-10 for any R, so every
+    history is bit-identical however the slab is cut; the exact tallies,
+    summed over all cycles and concatenated over ranks, must equal the
+    one-layer run's bit for bit, and every particle must end disabled."""
+    w1, d1, e1, c1 = _sync_totals(oracle, 1)
+    assert d1 == 20_000 and c1 == 40          # 20,000 particles at 500 per cycle
+    for R in (2, 3, 5, 8):
+        w, d, e, c = _sync_totals(oracle, R)
+        assert d == 20_000 and e == e1
+        assert np.array_equal(w.view(np.uint64), w1.view(np.uint64)), R
+        assert c > c1                          # escapes cost extra cycles
+
+
+def test_sync_cycle_loop_pin_catches_wrong_exchange(oracle, tmp_path):
+    lib = oracle.build_mutant(9, str(tmp_path))
+    with oracle.using(lib):
+        w1, _, _, _ = _sync_totals(oracle, 1, n=2_000)
+        with pytest.raises(RuntimeError):
+            _sync_totals(oracle, 4, n=2_000, max_cycles=2_000)
+
+
+@needs_ref
+def test_single_rank_cycle_loop_matches_reference_layer(oracle):
+    """One rank: the loop is Layer::simulate(nb_particles_per_cycle) until all
+    particles are disabled; the FP32 oracle against the reference's own Layer
+    on the reference's config.yaml source (x_ini 0.70710677, 1000 cells)."""
+    n, per = 5_000, 500
+    layers, cycles = oracle.sync_run(32, 0.0, 1.0, np.float32(0.7071067690849304), 1, 1000, n,
+                                     np.float32(1e-12), per)
+    R = oracle.RefLayer.decompose_domain(0.0, 1.0, np.float32(0.7071067690849304), 1, 0, 1000, n,
+                                         np.float32(1e-12))
+    ref_cycles = 0
+    while R.nb_disabled < n:
+        R.simulate(per, 1)
+        ref_cycles += 1
+    assert ref_cycles == cycles
+    assert np.array_equal(layers[0].weights_absorbed.view(np.uint32),
+                          R.weights_absorbed.view(np.uint32))
