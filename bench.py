@@ -17,7 +17,9 @@ Also reported, in "variants":
                slab ("per-problem launches") next to its batched time;
   adjoint   -- config 5 (1,000 cells, 1e7 particles): forward with
                per-history scores + seed-replay adjoint, against the plain
-               forward, for d(absorbed weight in cells 800-899)/d(sigma).
+               forward, for d(absorbed weight in cells 800-899)/d(sigma);
+  adjoint_batched -- the same across 64 such slabs (6.4e8 particles) with
+               the batched forward and adjoint launches.
 Each variant's events are device-counted.
 
 Prints one JSON line.  --gpus N > 1 is refused: this build runs on one GPU.
@@ -27,6 +29,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "mc-mpi_amd"))
@@ -45,18 +49,24 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def device_bank(ctx, n, mu_fn, torch):
+def device_bank(ctx, n, mu_seed, torch):
     """A source bank built on the device: the reference seed chain by
-    jump-ahead (mcgpu_rng_seed_chain), x = 0, index = 0, weight 1, directions
-    from mu_fn(count, offset) (the generators of mcgpu.configs)."""
+    jump-ahead (mcgpu_rng_seed_chain), x = 0, index = 0, weight 1, and
+    directions mu = sqrt(uniform(mu_seed, .)) from the splitmix64 generator of
+    mcgpu.configs evaluated with torch on the device (a prefix is checked
+    bit for bit against the host generator)."""
     from mcgpu import configs
     seed = torch.empty(n, dtype=torch.int64, device="cuda")
     ctx.rng_seed_chain(configs.ROOT_SEED, seed)
     mu = torch.empty(n, dtype=torch.float64, device="cuda")
-    chunk = 20_000_000
+    chunk = 50_000_000
     for o in range(0, n, chunk):
         k = min(chunk, n - o)
-        mu[o:o + k] = torch.from_numpy(mu_fn(k, o)).cuda()
+        mu[o:o + k] = torch.sqrt(configs.uniform_torch(mu_seed, k, o))
+    k = min(n, 100_000)
+    want = np.sqrt(configs.uniform(mu_seed, k))
+    if not np.array_equal(mu[:k].cpu().numpy().view(np.uint64), want.view(np.uint64)):
+        raise RuntimeError("device-generated directions differ from the host generator")
     return {"seed": seed, "x": torch.zeros(n, dtype=torch.float64, device="cuda"), "mu": mu,
             "wmc": torch.ones(n, dtype=torch.float64, device="cuda"),
             "index": torch.zeros(n, dtype=torch.int32, device="cuda")}
@@ -76,7 +86,6 @@ def cpu_baselines(n_ref, n_port):
                    per-history results equal the reference's);
       port      -- the FP64 oracle (the same arithmetic as the GPU path).
     Returns (reference or None when oracle/_ref is not built, port)."""
-    import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from mcgpu import configs
@@ -126,7 +135,7 @@ def run_plain(ctx, args, torch, mcgpu, configs):
     n = args.particles
     wl = configs.config2(n, with_bank=False)
     log(f"config 2: building the {n}-particle bank on the device")
-    bank0 = device_bank(ctx, n, lambda k, o: configs.config2_mu(k, offset=o), torch)
+    bank0 = device_bank(ctx, n, 2 * 1000 + 3, torch)  # config2_mu
     bank = {k: v.clone() for k, v in bank0.items()}
     sigs = torch.from_numpy(wl.sigs).cuda()
     ar = torch.from_numpy(wl.absorption_rates).cuda()
@@ -172,7 +181,7 @@ def run_plain(ctx, args, torch, mcgpu, configs):
 def run_batched(ctx, args, torch, mcgpu, configs):
     log("config 4: layout and bank")
     slabs, sigs, ar, total = configs.config4_layout(particle_scale=args.c4_scale)
-    bank0 = device_bank(ctx, total, lambda k, o: configs.config4_mu(k, offset=o), torch)
+    bank0 = device_bank(ctx, total, 4 * 1000 + 4, torch)  # config4_mu
     bank = {k: v.clone() for k, v in bank0.items()}
     d_sigs = torch.from_numpy(sigs).cuda()
     d_ar = torch.from_numpy(ar).cuda()
@@ -265,6 +274,40 @@ def run_adjoint(ctx, args, torch, mcgpu, configs):
     return res
 
 
+def run_adjoint_batched(ctx, args, torch, mcgpu, configs):
+    """Config 5 batched across 64 slabs: batched forward with scores +
+    batched adjoint against the batched plain forward."""
+    slabs, sa, ss, g, total = configs.config5_batch(64, args.c5b_particles)
+    log(f"config 5 batched: {len(slabs)} slabs, {total} particles")
+    init = device_bank(ctx, total, 5004, torch)  # config5_batch_mu
+    d_sa, d_ss, d_g = (torch.from_numpy(v).cuda() for v in (sa, ss, g))
+    w = torch.zeros(len(sa), dtype=torch.float64, device="cuda")
+    work = {k: v.clone() for k, v in init.items()}
+    plain = ctx.forward_cells_batch(slabs, work, d_sa, d_ss, w)
+    for k in work:
+        work[k].copy_(init[k])
+    w.zero_()
+    score = torch.empty(total, dtype=torch.float64, device="cuda")
+    fwd = ctx.forward_cells_batch(slabs, work, d_sa, d_ss, w, g=d_g, history_score=score)
+    del work
+    ga = torch.zeros(len(sa), dtype=torch.float64, device="cuda")
+    gi = torch.zeros(len(sa), dtype=torch.float64, device="cuda")
+    adj = ctx.adjoint_cells_batch(slabs, init, d_sa, d_ss, d_g, ga, gi, history_score=score)
+    out = {"workload": f"config5 x64: 64 slabs of 1000 cells, {total} particles, "
+                       "target cells 800-899 of every slab",
+           "forward_ms": plain["kernel_ms"], "forward_events": plain["events"],
+           "forward_scored_ms": fwd["kernel_ms"], "adjoint_ms": adj["kernel_ms"],
+           "fwd_plus_adjoint_over_forward": (fwd["kernel_ms"] + adj["kernel_ms"]) /
+           plain["kernel_ms"],
+           "events_per_s_forward": plain["events"] / plain["kernel_ms"] * 1e3,
+           "events_per_s_fwd_plus_adjoint": plain["events"] /
+           (fwd["kernel_ms"] + adj["kernel_ms"]) * 1e3}
+    log(f"adjoint batched: {out}")
+    del init, score
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -282,6 +325,8 @@ def main():
                     help="particles for the FP64 oracle CPU leg")
     ap.add_argument("--c4-scale", type=float, default=1.0, help="config 4 particle scale")
     ap.add_argument("--c5-particles", type=int, default=10_000_000)
+    ap.add_argument("--c5b-particles", type=int, default=10_000_000,
+                    help="particles per slab of the batched config 5")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-only", action="store_true",
                     help="only the config-2 steps (for profiling runs)")
@@ -344,6 +389,7 @@ def main():
     if not args.plain_only:
         out["variants"]["batched"] = run_batched(ctx, args, torch, mcgpu, configs)
         out["variants"]["adjoint"] = run_adjoint(ctx, args, torch, mcgpu, configs)
+        out["variants"]["adjoint_batched"] = run_adjoint_batched(ctx, args, torch, mcgpu, configs)
     if not args.no_cpu:
         log("timing the CPU baselines")
         ref, port = cpu_baselines(args.cpu_ref_sample, args.cpu_port_sample)

@@ -229,6 +229,29 @@ int mcgpu_adjoint_cells(mcgpu_context *ctx, int64_t n, const mcgpu_bank *initial
                         double *grad_sig_i, const mcgpu_options *opt, mcgpu_stats *stats,
                         void *stream);
 
+/*
+ * Batched forms of the two calls above: many independent slabs (the
+ * mcgpu_slab layout of mcgpu_batch_create) in one persistent launch each.
+ * sig_a, sig_i, g, weights_absorbed, grad_sig_a and grad_sig_i are
+ * concatenations over the slabs' cells (slab s owns [cell_offset, +m)), and
+ * bank / initial, history_score and result over their particles (slab s owns
+ * [particle_offset, +n)).  Every slab's outputs are bit-identical to the
+ * single-slab call on that slab.  options: block_size, grid_blocks,
+ * events_per_refill, particles_per_lane (tile = particles_per_lane x
+ * block_size histories).
+ */
+int mcgpu_forward_cells_batch(mcgpu_context *ctx, int32_t n_slabs, const mcgpu_slab *slabs,
+                              int64_t total_cells, const mcgpu_bank *bank, const double *sig_a,
+                              const double *sig_i, double *weights_absorbed, const double *g,
+                              double *history_score, int8_t *result, const mcgpu_options *opt,
+                              mcgpu_stats *stats, void *stream);
+int mcgpu_adjoint_cells_batch(mcgpu_context *ctx, int32_t n_slabs, const mcgpu_slab *slabs,
+                              int64_t total_cells, const mcgpu_bank *initial,
+                              const double *sig_a, const double *sig_i, const double *g,
+                              const double *history_score, double *grad_sig_a,
+                              double *grad_sig_i, const mcgpu_options *opt, mcgpu_stats *stats,
+                              void *stream);
+
 /* ------------------------------------------------------------- RNG helpers
  * Known-answer helpers for the reference RNG (src/random.cpp), evaluated on
  * the device (all pointers are device pointers):

@@ -120,16 +120,11 @@ __global__ __launch_bounds__(1024) void k_adjoint(adj_args a, int refill_k) {
           continue;
         }
         const double gc = g[c];
-        const double gdw = gc * dw;
-        const double t_from = T - P;
-        const double a_after = t_from - gdw;
-        const double term_a = di * (gc * p.wmc - a_after);
+        double term_a, term_i;
+        const bool has_i = mc_adjoint_terms(cell, coll, di, dw, gc, p.wmc, T, P, term_a, term_i);
         mc_tally(tl, tl + m, a.flags, &a.ctr->status, c, term_a);
-        if (cell.sig_i > MC_EPS) {
-          const double s = coll ? (1.0 / cell.sig_i) - di : -di;
-          mc_tally(tl + 2 * m, tl + 3 * m, a.flags + m, &a.ctr->status, c, s * t_from);
-        }
-        P = P + gdw;
+        if (has_i) mc_tally(tl + 2 * m, tl + 3 * m, a.flags + m, &a.ctr->status, c, term_i);
+        P = P + gc * dw;
       }
       const bool alive = mc_alive(p, a.L);
       if (!alive || ev >= max_ev) {

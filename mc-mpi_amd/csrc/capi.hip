@@ -275,7 +275,7 @@ int mcgpu_context_destroy(mcgpu_context *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   void *bufs[] = {ctx->cells, ctx->fx, ctx->flags, ctx->counters, ctx->aos,
-                  ctx->res32, ctx->sig_in, ctx->w_in, ctx->work};
+                  ctx->res32, ctx->sig_in, ctx->w_in, ctx->work, ctx->tables};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (ctx->host_counters) (void)hipHostFree(ctx->host_counters);

@@ -45,6 +45,7 @@ struct mcgpu_context {
   void *sig_in = nullptr;     size_t sig_in_bytes = 0;    /* double[2m] sigs, ar */
   void *w_in = nullptr;       size_t w_in_bytes = 0;      /* double[m] weights */
   void *work = nullptr;       size_t work_bytes = 0;      /* event kernel banks */
+  void *tables = nullptr;     size_t tables_bytes = 0;    /* batched calls: slab + tile tables */
 };
 
 /* grow-only device buffer */

@@ -238,4 +238,27 @@ __device__ __forceinline__ void mc_tally(unsigned long long *lo, unsigned long l
   mc_fx_atomic_add(&lo[c], &hi[c], v);
 }
 
+/* Gradient terms of one replayed event for the target T = sum g_c tally_c
+   (cell c, flight length di, collision flag, deposit dw, weight wmc after the
+   event, history total T and partial sum P = sum of g dw before the event):
+     pathwise, sig_a:       di * (g_c * wmc - (T - P - g_c * dw))
+     score function, sig_i: ([coll] / sig_i - di) * (T - P), only where
+                            sig_i > EPS_PRECISION (sampled flight length).
+   Returns whether the sig_i term exists. */
+__device__ __forceinline__ bool mc_adjoint_terms(const mc_cell cell, bool coll, double di,
+                                                 double dw, double gc, double wmc, double T,
+                                                 double P, double &term_a, double &term_i) {
+  const double gdw = gc * dw;
+  const double t_from = T - P;
+  const double a_after = t_from - gdw;
+  term_a = di * (gc * wmc - a_after);
+  if (cell.sig_i > MC_EPS) {
+    const double s = coll ? (1.0 / cell.sig_i) - di : -di;
+    term_i = s * t_from;
+    return true;
+  }
+  term_i = 0.0;
+  return false;
+}
+
 #endif /* MC_DEVICE_H */

@@ -59,6 +59,11 @@ def _ptr(t):
     return t.data_ptr()
 
 
+def _bank_struct(bank):
+    return Bank(_ptr(bank["seed"]), _ptr(bank["x"]), _ptr(bank["mu"]), _ptr(bank["wmc"]),
+                _ptr(bank["index"]))
+
+
 def bank_to_device(particles, device="cuda"):
     """numpy PARTICLE array -> dict of torch tensors (SoA bank) on `device`."""
     import torch
@@ -188,6 +193,36 @@ class Context:
                                        _ptr(g), _ptr(history_score), _ptr(grad_sig_a),
                                        _ptr(grad_sig_i), C.byref(opt), C.byref(st),
                                        _stream(stream))
+        self.check(rc)
+        return st.as_dict()
+
+    def forward_cells_batch(self, slabs, bank, sig_a, sig_i, weights_absorbed, g=None,
+                            history_score=None, result=None, options=None, stream=None):
+        """mcgpu_forward_cells_batch over slabs (tuples as in Batch)."""
+        arr = (Slab * len(slabs))(*[Slab(*s) for s in slabs])
+        b = _bank_struct(bank)
+        st = Stats()
+        opt = options if options is not None else make_options()
+        rc = lib().mcgpu_forward_cells_batch(self._h, len(slabs), C.cast(arr, C.c_void_p),
+                                             sig_a.numel(), C.byref(b), _ptr(sig_a), _ptr(sig_i),
+                                             _ptr(weights_absorbed), _ptr(g), _ptr(history_score),
+                                             _ptr(result), C.byref(opt), C.byref(st),
+                                             _stream(stream))
+        self.check(rc)
+        return st.as_dict()
+
+    def adjoint_cells_batch(self, slabs, initial_bank, sig_a, sig_i, g, grad_sig_a, grad_sig_i,
+                            history_score=None, options=None, stream=None):
+        """mcgpu_adjoint_cells_batch over slabs (tuples as in Batch)."""
+        arr = (Slab * len(slabs))(*[Slab(*s) for s in slabs])
+        b = _bank_struct(initial_bank)
+        st = Stats()
+        opt = options if options is not None else make_options()
+        rc = lib().mcgpu_adjoint_cells_batch(self._h, len(slabs), C.cast(arr, C.c_void_p),
+                                             sig_a.numel(), C.byref(b), _ptr(sig_a), _ptr(sig_i),
+                                             _ptr(g), _ptr(history_score), _ptr(grad_sig_a),
+                                             _ptr(grad_sig_i), C.byref(opt), C.byref(st),
+                                             _stream(stream))
         self.check(rc)
         return st.as_dict()
 

@@ -86,6 +86,10 @@ SIGNATURES = {
                                   vp, vp, C.POINTER(Options), C.POINTER(Stats), vp]),
     "mcgpu_adjoint_cells": (i32, [vp, i64, C.POINTER(Bank), vp, vp, i32, i32, dbl, dbl, vp, vp,
                                   vp, vp, C.POINTER(Options), C.POINTER(Stats), vp]),
+    "mcgpu_forward_cells_batch": (i32, [vp, i32, vp, i64, C.POINTER(Bank), vp, vp, vp, vp, vp,
+                                        vp, C.POINTER(Options), C.POINTER(Stats), vp]),
+    "mcgpu_adjoint_cells_batch": (i32, [vp, i32, vp, i64, C.POINTER(Bank), vp, vp, vp, vp, vp,
+                                        vp, C.POINTER(Options), C.POINTER(Stats), vp]),
     "mcgpu_rng_real_step": (i32, [vp, i64, vp, vp, vp]),
     "mcgpu_rng_seed_chain": (i32, [vp, u64, i64, vp, vp]),
     "mcgpu_math_eval": (i32, [vp, C.c_int, i64, vp, vp, vp]),

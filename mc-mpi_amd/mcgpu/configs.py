@@ -45,6 +45,29 @@ def uniform(seed, n, offset=0):
     return (splitmix64(seed, n, offset) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
 
 
+def _u64(v):
+    """uint64 constant as the int64 with the same bits (torch has no uint64 arithmetic)."""
+    return v - (1 << 64) if v >= 1 << 63 else v
+
+
+def _srl(z, k):
+    """Logical right shift of int64 tensors holding uint64 bits."""
+    return (z >> k) & ((1 << (64 - k)) - 1)
+
+
+def uniform_torch(seed, n, offset=0, device="cuda"):
+    """uniform(seed, n, offset) computed with torch on `device`: the same
+    splitmix64 outputs in wrapping int64 arithmetic (two's complement, so the
+    bits equal the uint64 ones), the same (x >> 11) * 2^-53."""
+    import torch
+    i = torch.arange(offset + 1, offset + n + 1, dtype=torch.int64, device=device)
+    z = _u64(seed) + i * _u64(0x9E3779B97F4A7C15)
+    z = (z ^ _srl(z, 30)) * _u64(0xBF58476D1CE4E5B9)
+    z = (z ^ _srl(z, 27)) * _u64(0x94D049BB133111EB)
+    z = z ^ _srl(z, 31)
+    return _srl(z, 11).to(torch.float64) * 2.0 ** -53
+
+
 def lognormal_cells(seed, n, mu, sigma):
     """n log-normal(mu, sigma) values via Box-Muller with Python math."""
     u = uniform(seed, 2 * n)
@@ -188,6 +211,32 @@ def config5(n=10_000_000, seed=5):
     wl.target = np.zeros(m)
     wl.target[800:900] = 1.0
     return wl
+
+
+def config5_batch(n_slabs=64, n_per_slab=10_000_000):
+    """Config 5 batched: n_slabs independent 1000-cell slabs on [0,1]; slab s
+    has its own LogNormal(0, 0.5) sigma_a, sigma_s (seeds 5000*s + 5001 and
+    5000*s + 5002, so slab 0 has config 5's cross sections) and n_per_slab
+    particles at x = 0, weight 1, minimum weight 1e-3; the target is cells
+    800-899 of every slab.  Particle i of the concatenated bank takes output
+    i+1 of the reference seed chain and mu = sqrt(uniform(5004, .)[i]).
+    Returns (slabs, sigma_a, sigma_s, g, total_particles) in the layout of
+    mcgpu_forward_cells_batch (slab tuples as in config4_layout)."""
+    m = 1000
+    slabs, sa, ss, g = [], [], [], []
+    for s in range(n_slabs):
+        sa.append(lognormal_cells(5000 * s + 5001, m, 0.0, 0.5))
+        ss.append(lognormal_cells(5000 * s + 5002, m, 0.0, 0.5))
+        t = np.zeros(m)
+        t[800:900] = 1.0
+        g.append(t)
+        slabs.append((0, m, 1.0 / m, 1e-3, s * m, s * n_per_slab, n_per_slab))
+    return slabs, np.concatenate(sa), np.concatenate(ss), np.concatenate(g), n_slabs * n_per_slab
+
+
+def config5_batch_mu(n, offset=0):
+    """Directions of the batched config 5's particles offset .. offset+n-1."""
+    return np.sqrt(uniform(5004, n, offset))
 
 
 def config4_layout(n_slabs=4096, seed=4, particle_scale=1.0):

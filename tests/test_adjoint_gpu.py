@@ -154,3 +154,96 @@ def test_autograd_function_matches_adjoint(ctx):
         assert torch.equal(t_sa.grad, ga) and torch.equal(t_ss.grad, gi)
     # the problem's bank is the untouched initial bank
     assert np.array_equal(mcgpu.bank_from_device(bank)["wmc"], wl.bank["wmc"])
+
+
+def _slab_batch(rng_seed=9):
+    """Six independent slabs of uneven size (one with cells beyond the LDS
+    budget, one with no particles, one not starting at cell 0), concatenated
+    as the batched entry points take them."""
+    from mcgpu import configs
+    rng = np.random.default_rng(rng_seed)
+    shapes = [(1000, 0, 4000), (40, 0, 3000), (5000, 100, 2500), (300, 7, 0), (1000, 0, 3500),
+              (777, 0, 1500)]  # (cells, index_start, particles)
+    slabs, sa, ss, g, banks = [], [], [], [], []
+    coff = poff = 0
+    for m, start, n in shapes:
+        sa.append(rng.lognormal(0.0, 0.5, m))
+        ss.append(rng.lognormal(0.0, 0.5, m))
+        gg = np.zeros(m)
+        gg[int(0.8 * m):int(0.9 * m)] = 1.0
+        g.append(gg)
+        b = np.zeros(n, dtype=configs.PARTICLE)
+        b["seed"] = configs.seed_chain(n, start=poff)
+        b["x"] = start * (1.0 / m)
+        b["index"] = start
+        b["mu"] = np.sqrt(rng.uniform(0.0, 1.0, n))
+        b["wmc"] = 1.0
+        banks.append(b)
+        slabs.append((start, start + m, 1.0 / m, 1e-3, coff, poff, n))
+        coff += m
+        poff += n
+    return (slabs, np.concatenate(sa), np.concatenate(ss), np.concatenate(g),
+            np.concatenate(banks))
+
+
+def test_batched_forward_and_adjoint_equal_single_slab(ctx, oracle):
+    """mcgpu_forward_cells_batch / mcgpu_adjoint_cells_batch (with and without
+    stored scores): every slab's final bank, exit codes, tallies, scores and
+    gradients bit-identical to the single-slab calls, and one slab to the
+    oracle."""
+    import torch
+    import mcgpu
+    slabs, sa, ss, g, bank = _slab_batch()
+    d_sa, d_ss, d_g = _dev(sa), _dev(ss), _dev(g)
+    total = len(sa)
+    # single-slab references
+    ref = {}
+    for k, (lo, hi, dx, mw, coff, poff, n) in enumerate(slabs):
+        m = hi - lo
+        sl = slice(coff, coff + m)
+        b = mcgpu.bank_to_device(bank[poff:poff + n])
+        w = torch.zeros(m, dtype=torch.float64, device="cuda")
+        sc = torch.empty(n, dtype=torch.float64, device="cuda")
+        r = torch.empty(n, dtype=torch.int8, device="cuda")
+        if n:
+            ctx.forward_cells(b, d_sa[sl], d_ss[sl], w, lo, hi, dx, mw, g=d_g[sl],
+                              history_score=sc, result=r)
+        ga = torch.zeros(m, dtype=torch.float64, device="cuda")
+        gi = torch.zeros(m, dtype=torch.float64, device="cuda")
+        if n:
+            ctx.adjoint_cells(mcgpu.bank_to_device(bank[poff:poff + n]), d_sa[sl], d_ss[sl], lo, hi,
+                              dx, mw, d_g[sl], ga, gi, history_score=sc)
+        ref[k] = (mcgpu.bank_from_device(b), r.cpu().numpy(), w.cpu().numpy(), sc.cpu().numpy(),
+                  ga.cpu().numpy(), gi.cpu().numpy())
+    # batched forward with scores
+    d_bank = mcgpu.bank_to_device(bank)
+    w = torch.zeros(total, dtype=torch.float64, device="cuda")
+    sc = torch.empty(len(bank), dtype=torch.float64, device="cuda")
+    r = torch.full((len(bank),), 9, dtype=torch.int8, device="cuda")
+    st = ctx.forward_cells_batch(slabs, d_bank, d_sa, d_ss, w, g=d_g, history_score=sc, result=r)
+    _say(f"batched forward: {st['events']} events")
+    fin, r, w, scn = mcgpu.bank_from_device(d_bank), r.cpu().numpy(), w.cpu().numpy(), sc.cpu().numpy()
+    for with_score in (True, False):
+        ga = torch.zeros(total, dtype=torch.float64, device="cuda")
+        gi = torch.zeros(total, dtype=torch.float64, device="cuda")
+        ctx.adjoint_cells_batch(slabs, mcgpu.bank_to_device(bank), d_sa, d_ss, d_g, ga, gi,
+                                history_score=sc if with_score else None)
+        gan, gin = ga.cpu().numpy(), gi.cpu().numpy()
+        for k, (lo, hi, dx, mw, coff, poff, n) in enumerate(slabs):
+            rb, rr, rw, rs, rga, rgi = ref[k]
+            sl, ps = slice(coff, coff + hi - lo), slice(poff, poff + n)
+            for f in ("seed", "x", "mu", "wmc", "index"):
+                assert np.array_equal(fin[f][ps], rb[f]), (k, f)
+            assert np.array_equal(r[ps], rr) and np.array_equal(w[sl], rw), k
+            assert np.array_equal(scn[ps].view(np.uint64), rs.view(np.uint64)), k
+            assert np.array_equal(gan[sl].view(np.uint64), rga.view(np.uint64)), (k, with_score)
+            assert np.array_equal(gin[sl].view(np.uint64), rgi.view(np.uint64)), (k, with_score)
+    # one slab against the oracle
+    lo, hi, dx, mw, coff, poff, n = slabs[2]
+    sl = slice(coff, coff + hi - lo)
+    o = oracle.forward_cells(lo, hi - lo, dx, mw, sa[sl], ss[sl], bank[poff:poff + n], g=g[sl])
+    assert np.array_equal(w[sl], o["tally_exact"])
+    oa, oi = oracle.adjoint_cells(lo, hi - lo, dx, mw, sa[sl], ss[sl], bank[poff:poff + n], g[sl],
+                                  score=o["score"])
+    assert np.array_equal(gan[sl].view(np.uint64), oa.view(np.uint64))
+    assert np.array_equal(gin[sl].view(np.uint64), oi.view(np.uint64))

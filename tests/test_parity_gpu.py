@@ -92,6 +92,20 @@ def test_device_rng_known_answers(ctx, oracle):
     assert np.array_equal(reals.cpu().numpy().view(np.uint64), r.view(np.uint64))
 
 
+def test_device_uniform_generator_and_sqrt():
+    """The benchmark builds source directions on the device with
+    mcgpu.configs.uniform_torch and torch.sqrt: both must equal the host
+    generator bit for bit."""
+    import torch
+    from mcgpu import configs
+    for seed, offset in ((2003, 0), (4004, 300_000_000)):
+        u = configs.uniform_torch(seed, 500_000, offset)
+        want = configs.uniform(seed, 500_000, offset)
+        assert np.array_equal(u.cpu().numpy().view(np.uint64), want.view(np.uint64))
+        assert np.array_equal(torch.sqrt(u).cpu().numpy().view(np.uint64),
+                              np.sqrt(want).view(np.uint64))
+
+
 def test_device_create_particles(ctx, oracle):
     """mcgpu_create_particles against the oracle's Layer: decompose_domain with
     the reference's config.yaml source (x_ini = 0.7071..., 1000 cells, seed
