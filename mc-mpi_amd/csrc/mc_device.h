@@ -64,19 +64,29 @@ struct mc_fx {
   uint64_t hi;
 };
 
-__device__ __forceinline__ mc_fx mc_fx_from_double(double v, uint32_t &flags) {
+/* The conversion for every term (zero, subnormal, negative, large and
+   non-finite ones included), with its flags. */
+struct mc_fx_flagged {
+  mc_fx v;
+  uint32_t flags;
+};
+
+static __device__ __noinline__ mc_fx_flagged mc_fx_from_double_general(double v) {
+  mc_fx_flagged out;
+  out.flags = 0;
   uint64_t b = mc_asu64(v);
   uint32_t E = (uint32_t)(b >> 52) & 0x7FFu;
   mc_fx r = {0, 0};
+  out.v = r;
   if (E == 0x7FFu) {
-    flags |= (b & 0x000FFFFFFFFFFFFFULL) ? 1u : ((b >> 63) ? 4u : 2u);
-    return r;
+    out.flags = (b & 0x000FFFFFFFFFFFFFULL) ? 1u : ((b >> 63) ? 4u : 2u);
+    return out;
   }
   if (E >= 1023u + 39u) {
-    flags |= 8u;
-    return r;
+    out.flags = 8u;
+    return out;
   }
-  if (E == 0u) return r; /* zero or subnormal: below 2^-89 -> 0 */
+  if (E == 0u) return out; /* zero or subnormal: below 2^-89 -> 0 */
   uint64_t M = (b & 0x000FFFFFFFFFFFFFULL) | 0x0010000000000000ULL;
   int s = (int)E - 987; /* v = M * 2^(E-1075); v*2^88 = M * 2^(E-987) */
   if (s >= 64) {
@@ -93,7 +103,27 @@ __device__ __forceinline__ mc_fx mc_fx_from_double(double v, uint32_t &flags) {
     r.lo = ~r.lo + 1;
     r.hi = ~r.hi + (r.lo == 0 ? 1 : 0);
   }
-  return r;
+  out.v = r;
+  return out;
+}
+
+__device__ __forceinline__ mc_fx mc_fx_from_double(double v, uint32_t &flags) {
+  /* common case: 2^-35 <= v < 2^28, positive (every absorbed weight of a
+     history that starts at weight <= 1, except slivers below 2^-35): then
+     v * 2^88 = M * 2^s with 0 < s < 64 and the 128-bit integer is two
+     shifts of the 53-bit significand, exactly as in the general path */
+  const uint64_t b = mc_asu64(v);
+  const int s = (int)((uint32_t)(b >> 52)) - 987; /* the sign bit makes s >= 1061 */
+  if ((unsigned)(s - 1) < 63u) {
+    const uint64_t M = (b & 0x000FFFFFFFFFFFFFULL) | 0x0010000000000000ULL;
+    mc_fx r;
+    r.lo = M << s;
+    r.hi = M >> (64 - s);
+    return r;
+  }
+  const mc_fx_flagged g = mc_fx_from_double_general(v);
+  flags |= g.flags;
+  return g.v;
 }
 
 /* Correctly rounded conversion of the 128-bit sum back to double. */

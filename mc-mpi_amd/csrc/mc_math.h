@@ -119,9 +119,12 @@ MC_FN_COLD double mc_log_accurate(int e, double r, int idx) {
   }
   double lh, ll; /* log1p(r) = y * r */
   mc_dd_mul(yh, yl, r, 0.0, &lh, &ll);
-  /* + (-log c) */
+  /* + (-log c) = head + mid + lo (|head| >= |mid| or head == 0) */
+  double th, tl;
+  mc_fast_two_sum(mc_log_tab[idx][1], mc_log_tab[idx][2], &th, &tl);
+  tl = tl + mc_log_tab[idx][3];
   double sh, sl;
-  mc_dd_add(lh, ll, mc_log_tab[idx][1], mc_log_tab[idx][2], &sh, &sl);
+  mc_dd_add(lh, ll, th, tl, &sh, &sl);
   /* + e*ln2 (three-piece ln2; e*MC_LN2_HI is exact) */
   double ed = (double)e;
   double kh = ed * MC_LN2_HI;
@@ -171,14 +174,18 @@ MC_FN double mc_log(double x) {
   p = mc_fma(r, p, mc_inv_n[3][0]);
   double t = (r2h * r) * p;
 
-  /* e*ln2 + (-log c) + r - r^2/2 + t, accumulated in double-double */
+  /* e*ln2 + (-log c) + r - r^2/2 + t, accumulated in double-double.
+     s = e*ln2_hi + head is exact (both on the 2^-42 grid, |s| < 2^10); the
+     next two sums are FastTwoSums: |s| >= |r| unless s == 0 (e != 0 gives
+     |s| > 0.34; e == 0 is checked per table row by gen_math_tables.py), and
+     s + r then dominates r^2/2. */
   double ed = (double)e;
-  double s, serr, s2, serr2, s3, serr3;
-  mc_two_sum(ed * MC_LN2_HI, mc_log_tab[idx][1], &s, &serr);
-  double lo = serr + (ed * MC_LN2_MID + mc_log_tab[idx][2]);
-  mc_two_sum(s, r, &s2, &serr2);
+  double s2, serr2, s3, serr3;
+  const double s = ed * MC_LN2_HI + mc_log_tab[idx][1];
+  double lo = ed * MC_LN2_MID + mc_log_tab[idx][2];
+  mc_fast_two_sum(s, r, &s2, &serr2);
   lo = lo + serr2;
-  mc_two_sum(s2, -0.5 * r2h, &s3, &serr3);
+  mc_fast_two_sum(s2, -0.5 * r2h, &s3, &serr3);
   lo = lo + (serr3 + (t - 0.5 * r2l));
   double hi;
   mc_fast_two_sum(s3, lo, &hi, &lo);
