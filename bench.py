@@ -330,6 +330,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-only", action="store_true",
                     help="only the config-2 steps (for profiling runs)")
+    ap.add_argument("--skip", default="",
+                    help="comma-separated variants to leave out (batched, adjoint, adjoint_batched)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (untimed), the "
                          "known-size reference kernel for HBM byte counters")
@@ -387,9 +389,12 @@ def main():
                              "traffic": None if tr_o is None else tr_o * ev_o}
         out["variants"][other["kernel"]] = other
     if not args.plain_only:
-        out["variants"]["batched"] = run_batched(ctx, args, torch, mcgpu, configs)
-        out["variants"]["adjoint"] = run_adjoint(ctx, args, torch, mcgpu, configs)
-        out["variants"]["adjoint_batched"] = run_adjoint_batched(ctx, args, torch, mcgpu, configs)
+        runs = {"batched": run_batched, "adjoint": run_adjoint,
+                "adjoint_batched": run_adjoint_batched}
+        for name, fn in runs.items():
+            if args.skip and name in args.skip.split(","):
+                continue
+            out["variants"][name] = fn(ctx, args, torch, mcgpu, configs)
     if not args.no_cpu:
         log("timing the CPU baselines")
         ref, port = cpu_baselines(args.cpu_ref_sample, args.cpu_port_sample)

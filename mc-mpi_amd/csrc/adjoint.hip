@@ -122,8 +122,8 @@ __global__ __launch_bounds__(1024) void k_adjoint(adj_args a, int refill_k) {
         const double gc = g[c];
         double term_a, term_i;
         const bool has_i = mc_adjoint_terms(cell, coll, di, dw, gc, p.wmc, T, P, term_a, term_i);
-        mc_tally(tl, tl + m, a.flags, &a.ctr->status, c, term_a);
-        if (has_i) mc_tally(tl + 2 * m, tl + 3 * m, a.flags + m, &a.ctr->status, c, term_i);
+        mc_tally<true>(tl, tl + m, a.flags, &a.ctr->status, c, term_a);
+        if (has_i) mc_tally<true>(tl + 2 * m, tl + 3 * m, a.flags + m, &a.ctr->status, c, term_i);
         P = P + gc * dw;
       }
       const bool alive = mc_alive(p, a.L);

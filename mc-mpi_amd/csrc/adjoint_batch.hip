@@ -326,8 +326,8 @@ __global__ __launch_bounds__(1024) void k_adjoint_cells_batch(bcell_args a, int 
           const double gc = g[c];
           double term_a, term_i;
           const bool has_i = mc_adjoint_terms(cell, coll, di, dw, gc, p.wmc, T, P, term_a, term_i);
-          mc_tally(ga_lo, ga_hi, fa, &a.ctr->status, c, term_a);
-          if (has_i) mc_tally(gi_lo, gi_hi, fi, &a.ctr->status, c, term_i);
+          mc_tally<true>(ga_lo, ga_hi, fa, &a.ctr->status, c, term_a);
+          if (has_i) mc_tally<true>(gi_lo, gi_hi, fi, &a.ctr->status, c, term_i);
           P = P + gc * dw;
         }
         const bool alive = mc_alive(p, L);

@@ -51,107 +51,7 @@ __device__ __forceinline__ mc_cell mc_make_cell(double sigs, double ar) {
   return c;
 }
 
-/* ------------------------------------------------- exact fixed-point tally
- * A term v maps to the signed 128-bit integer round_half_up(v * 2^88),
- * stored as (lo, hi) two's complement.  Non-finite terms and |v| >= 2^39 set
- * flag bits instead (1 NaN, 2 +inf, 4 -inf, 8 out of range).  Integer sums
- * are exact, so the tally does not depend on the order in which lanes, waves
- * and blocks add their terms. */
-#define MC_FX_FRAC 88
-
-struct mc_fx {
-  uint64_t lo;
-  uint64_t hi;
-};
-
-/* The conversion for every term (zero, subnormal, negative, large and
-   non-finite ones included), with its flags. */
-struct mc_fx_flagged {
-  mc_fx v;
-  uint32_t flags;
-};
-
-static __device__ __noinline__ mc_fx_flagged mc_fx_from_double_general(double v) {
-  mc_fx_flagged out;
-  out.flags = 0;
-  uint64_t b = mc_asu64(v);
-  uint32_t E = (uint32_t)(b >> 52) & 0x7FFu;
-  mc_fx r = {0, 0};
-  out.v = r;
-  if (E == 0x7FFu) {
-    out.flags = (b & 0x000FFFFFFFFFFFFFULL) ? 1u : ((b >> 63) ? 4u : 2u);
-    return out;
-  }
-  if (E >= 1023u + 39u) {
-    out.flags = 8u;
-    return out;
-  }
-  if (E == 0u) return out; /* zero or subnormal: below 2^-89 -> 0 */
-  uint64_t M = (b & 0x000FFFFFFFFFFFFFULL) | 0x0010000000000000ULL;
-  int s = (int)E - 987; /* v = M * 2^(E-1075); v*2^88 = M * 2^(E-987) */
-  if (s >= 64) {
-    r.hi = M << (s - 64);
-  } else if (s > 0) {
-    r.lo = M << s;
-    r.hi = M >> (64 - s);
-  } else if (s == 0) {
-    r.lo = M;
-  } else if (s > -64) {
-    r.lo = ((M >> (-s - 1)) + 1) >> 1; /* round half up */
-  }
-  if (b >> 63) { /* negate (two's complement) */
-    r.lo = ~r.lo + 1;
-    r.hi = ~r.hi + (r.lo == 0 ? 1 : 0);
-  }
-  out.v = r;
-  return out;
-}
-
-__device__ __forceinline__ mc_fx mc_fx_from_double(double v, uint32_t &flags) {
-  /* common case: 2^-35 <= v < 2^28, positive (every absorbed weight of a
-     history that starts at weight <= 1, except slivers below 2^-35): then
-     v * 2^88 = M * 2^s with 0 < s < 64 and the 128-bit integer is two
-     shifts of the 53-bit significand, exactly as in the general path */
-  const uint64_t b = mc_asu64(v);
-  const int s = (int)((uint32_t)(b >> 52)) - 987; /* the sign bit makes s >= 1061 */
-  if ((unsigned)(s - 1) < 63u) {
-    const uint64_t M = (b & 0x000FFFFFFFFFFFFFULL) | 0x0010000000000000ULL;
-    mc_fx r;
-    r.lo = M << s;
-    r.hi = M >> (64 - s);
-    return r;
-  }
-  const mc_fx_flagged g = mc_fx_from_double_general(v);
-  flags |= g.flags;
-  return g.v;
-}
-
-/* Correctly rounded conversion of the 128-bit sum back to double. */
-__host__ __device__ __forceinline__ double mc_fx_to_double(uint64_t lo, uint64_t hi) {
-  int neg = (int64_t)hi < 0;
-  if (neg) {
-    lo = ~lo + 1;
-    hi = ~hi + (lo == 0 ? 1 : 0);
-  }
-  double d;
-  if (hi == 0) {
-    d = (double)lo * 0x1p-88;
-  } else {
-    int lz = __builtin_clzll(hi);
-    uint64_t top = lz ? ((hi << lz) | (lo >> (64 - lz))) : hi;
-    uint64_t rest = lz ? (lo << lz) : lo;
-    top |= (rest != 0) ? 1u : 0u; /* sticky bit below the rounding position */
-    d = __builtin_ldexp((double)top, 64 - lz - MC_FX_FRAC);
-  }
-  return neg ? -d : d;
-}
-
-__host__ __device__ __forceinline__ double mc_fx_finalize(uint64_t lo, uint64_t hi, uint32_t flags) {
-  if ((flags & 1u) || (flags & 8u) || ((flags & 6u) == 6u)) return __builtin_nan("");
-  if (flags & 2u) return __builtin_inf();
-  if (flags & 4u) return -__builtin_inf();
-  return mc_fx_to_double(lo, hi);
-}
+#include "mc_fx.h"
 
 /* 128-bit add into two 64-bit words with 64-bit atomics (LDS or global):
    the carry out of the low word is detected from the returned old value. */
@@ -191,7 +91,11 @@ struct mc_layer_k {
 __device__ __forceinline__ double mc_event(mc_state &p, const mc_cell cell, const double dx,
                                            bool &collided, double &di_out) {
   const double h = mc_rnd_real(p.seed);
-  double di = cell.sig_i > MC_EPS ? -mc_log(h) / cell.sig_i : MC_MAXREAL;
+  /* -log(h) does not depend on the cell: computed unconditionally (it is a
+     pure function of h, drawn in every event), so the cell's load overlaps
+     with it instead of gating it */
+  const double nlog = -mc_log(h);
+  double di = cell.sig_i > MC_EPS ? nlog / cell.sig_i : MC_MAXREAL;
 
   int index_new;
   double x_new_edge;
@@ -257,10 +161,14 @@ __device__ __forceinline__ unsigned long long mc_wave_sum(unsigned long long v) 
 
 /* add one tally term (exact fixed point) to cell c of (lo, hi); non-finite or
    out-of-range terms set the cell's flag word and the call's status */
+/* SIGNED: terms of either sign are common (adjoint); otherwise negative terms
+   take the general conversion (same result, fewer instructions for the
+   transport's non-negative deposits). */
+template <bool SIGNED = false>
 __device__ __forceinline__ void mc_tally(unsigned long long *lo, unsigned long long *hi,
                                          uint32_t *flags, unsigned int *status, int c, double dw) {
   uint32_t f = 0;
-  const mc_fx v = mc_fx_from_double(dw, f);
+  const mc_fx v = mc_fx_from_double_p(dw, &f, SIGNED ? 0 : 1);
   if (f) {
     atomicOr(&flags[c], f);
     if (f & 8u) atomicOr(status, 2u);

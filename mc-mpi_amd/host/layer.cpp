@@ -187,6 +187,38 @@ void Layer::simulate(int nb_particles, int nthread, bool use_gpu) {
   }
 }
 
+WireParticle to_wire(const Particle &p) {
+  WireParticle w;
+  w.seed = p.seed;
+  w.x = (float)p.x;
+  w.mu = (float)p.mu;
+  w.wmc = (float)p.wmc;
+  w.index = p.index;
+  return w;
+}
+
+Particle from_wire(const WireParticle &w) {
+  Particle p{};
+  p.seed = w.seed;
+  p.x = w.x;
+  p.mu = w.mu;
+  p.wmc = w.wmc;
+  p.index = w.index;
+  return p;
+}
+
+std::vector<WireParticle> to_wire(const std::vector<Particle> &ps) {
+  std::vector<WireParticle> out(ps.size());
+  for (size_t i = 0; i < ps.size(); ++i) out[i] = to_wire(ps[i]);
+  return out;
+}
+
+std::vector<Particle> from_wire(const std::vector<WireParticle> &ws) {
+  std::vector<Particle> out(ws.size());
+  for (size_t i = 0; i < ws.size(); ++i) out[i] = from_wire(ws[i]);
+  return out;
+}
+
 /* src/layer.cpp:363-380 */
 void Layer::dump_WA() {
   FILE *file = fopen("WA.out", "w");

@@ -18,6 +18,7 @@
 #ifndef MCGPU_LAYER_HPP
 #define MCGPU_LAYER_HPP
 
+#include <cstddef>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -96,6 +97,31 @@ public:
 
 Layer decompose_domain(real_t x_min, real_t x_max, real_t x_ini, int world_size, int world_rank,
                        int nb_cells, int nb_particles, real_t particle_min_weight);
+
+/* The reference's Particle as it is built (real_t = float,
+   include/types/particle.hpp:7-17, include/types/types.hpp:12-17): 24 bytes,
+   the layout its MPI datatype describes (src/worker_sync.cpp:7-21) and its
+   communicators copy byte for byte.  For exchanging particles with ranks
+   that run the reference. */
+struct WireParticle {
+  seed_t seed;
+  float x;
+  float mu;
+  float wmc;
+  int index;
+};
+static_assert(sizeof(WireParticle) == 24, "reference Particle is 24 bytes");
+static_assert(offsetof(WireParticle, seed) == 0 && offsetof(WireParticle, x) == 8 &&
+                  offsetof(WireParticle, mu) == 12 && offsetof(WireParticle, wmc) == 16 &&
+                  offsetof(WireParticle, index) == 20,
+              "field offsets of the reference's MPI particle datatype");
+
+/* FP64 -> FP32 (x, mu, wmc rounded to nearest), seed and index unchanged */
+WireParticle to_wire(const Particle &p);
+/* FP32 -> FP64, exact */
+Particle from_wire(const WireParticle &w);
+std::vector<WireParticle> to_wire(const std::vector<Particle> &ps);
+std::vector<Particle> from_wire(const std::vector<WireParticle> &ws);
 
 /* src/random.cpp:24-29 */
 seed_t rnd_seed(seed_t *seed);

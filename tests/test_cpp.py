@@ -22,6 +22,23 @@ def test_yaml_loader(tmp_path):
     assert out.returncode == 0 and "all ok" in out.stdout
 
 
+def test_wire_particle_matches_reference_layout(tmp_path):
+    """The 24-byte FP32 Particle of the reference (its MPI datatype's layout):
+    the C++ WireParticle of the serialization literal of
+    src/test_async_serialization.cpp:29-31 has the bytes numpy packs for the
+    same record in the FP32 oracle's particle dtype, and FP32 -> FP64 -> FP32
+    is the identity."""
+    import numpy as np
+    import pyoracle
+    exe = os.path.join(BUILD, "test_wire")
+    assert os.path.exists(exe), "build first (__graft_entry__.build())"
+    out = _run([exe, str(tmp_path / "wire.bin")], str(tmp_path))
+    assert out.returncode == 0 and "all ok" in out.stdout
+    rec = np.zeros(1, pyoracle.P32)
+    rec[0] = (5127801, 0.172634512365276, 123.12342351, -1231.123486123, 2134512)
+    assert (tmp_path / "wire.bin").read_bytes() == rec.tobytes()
+
+
 @pytest.mark.gpu
 def test_layer_against_oracle(tmp_path):
     """mcgpu::Layer (GPU) vs the oracle's FP64 Layer: decompose_domain,
