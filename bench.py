@@ -10,6 +10,10 @@ restore of the bank from a pristine device copy.
 
 Also reported, in "variants":
   event     -- the event-based kernel on the same config-2 bank (one step);
+  host_api  -- mcgpu_simulate with a host bank (copies over PCIe included) on
+               the first 1e7 particles of config 2;
+  cycle     -- the reference's synchronous multi-rank loop with five layers
+               on this GPU (mcgpu_run --ranks 5), config.yaml source;
   batched   -- config 4 (4096 independent slabs, 5.8e8 particles) in one
                persistent launch replayed from a hipGraph, launch shape chosen
                by the autotuner on a quarter of the slabs; and the same
@@ -42,7 +46,7 @@ BYTES_PER_EVENT = 72  # algorithmic: the 36-byte particle state read + written p
 # this workload (1e7 particles), scaled by the copy-kernel calibration (read
 # x2.0, write x1.0); profiles/history_config2_counters.md and
 # profiles/event_config2_hbm_counters.md hold the counter runs.
-MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0604, "event": 88.1}
+MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0602, "event": 88.2}
 
 
 def log(msg):
@@ -176,6 +180,57 @@ def run_plain(ctx, args, torch, mcgpu, configs):
     del bank, bank0
     torch.cuda.empty_cache()
     return res, other
+
+
+def run_host_api(ctx, args, torch, mcgpu, configs):
+    """The drop-in path with host buffers (mcgpu_simulate: host AoS bank in,
+    copied to the device and back inside the call), timed on the wall clock
+    around the call, on the first 1e7 particles of config 2."""
+    n = args.host_particles
+    wl = configs.config2(n)
+    bank = wl.bank.copy()
+    w = np.zeros(wl.m)
+    t0 = time.perf_counter()
+    _, st, _ = ctx.simulate(bank, wl.sigs, wl.absorption_rates, w, wl.min_index, wl.max_index,
+                            wl.dx, wl.min_weight)
+    dt = time.perf_counter() - t0
+    out = {"workload": f"config2, first {n} particles, host AoS bank (40 B per particle)",
+           "events": st["events"], "wall_s": dt, "kernel_ms": st["kernel_ms"],
+           "events_per_s_pcie_inclusive": st["events"] / dt,
+           "events_per_s_kernel": st["events"] / st["kernel_ms"] * 1e3}
+    log(f"host api: {out}")
+    return out
+
+
+def run_cycle(ctx, args, torch, mcgpu, configs):
+    """`mcgpu_run config.yaml --ranks 5` (the synchronous multi-rank loop of
+    the reference's workers, all five layers on this GPU) on the reference's
+    config.yaml source with 1e6 particles and 1e5 per cycle; events / wall
+    seconds of the loop as mcgpu_run reports them."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "mc-mpi_amd", "mcgpu_run")
+    with tempfile.TemporaryDirectory() as d:
+        cfg = os.path.join(d, "config.yaml")
+        with open(cfg, "w") as f:
+            f.write("x_min: 0.000000000000000000e+00\nx_max: 1.000000000000000000e+00\n"
+                    "x_ini: 7.071067690849304199e-01\n"
+                    "particle_min_weight: 9.999999960041972003e-13\nnb_cells: 1000\n"
+                    f"nb_particles: {args.cycle_particles}\ncycle_time: 1.e-05\n"
+                    "nb_particles_per_cycle: 100000\nnthread: 1\nstatistics_cycle_time: 1.e-1\n")
+        r = subprocess.run([exe, cfg, "--ranks", "5"], cwd=d, capture_output=True, text=True,
+                           timeout=600)
+    if r.returncode != 0:
+        raise RuntimeError(f"mcgpu_run failed: {r.stderr[-500:]}")
+    secs = float(r.stdout.split()[0])
+    fields = dict(kv.split(" ", 1) for kv in r.stderr.strip().splitlines()[-1].split(", "))
+    events = int(fields["events"])
+    out = {"workload": f"config.yaml source, 1000 cells, {args.cycle_particles} particles, "
+                       "1e5 per cycle, 5 ranks on one GPU (mcgpu_run --ranks 5)",
+           "events": events, "cycles": int(fields["cycles"]), "wall_s": secs,
+           "events_per_s": events / secs}
+    log(f"cycle: {out}")
+    return out
 
 
 def run_batched(ctx, args, torch, mcgpu, configs):
@@ -331,7 +386,12 @@ def main():
     ap.add_argument("--plain-only", action="store_true",
                     help="only the config-2 steps (for profiling runs)")
     ap.add_argument("--skip", default="",
-                    help="comma-separated variants to leave out (batched, adjoint, adjoint_batched)")
+                    help="comma-separated variants to leave out (host_api, cycle, batched, "
+                         "adjoint, adjoint_batched)")
+    ap.add_argument("--cycle-particles", type=int, default=1_000_000,
+                    help="particles of the multi-rank cycle variant")
+    ap.add_argument("--host-particles", type=int, default=10_000_000,
+                    help="particles of the host-buffer (PCIe-inclusive) variant")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run one 1 GiB device-to-device copy first (untimed), the "
                          "known-size reference kernel for HBM byte counters")
@@ -389,8 +449,8 @@ def main():
                              "traffic": None if tr_o is None else tr_o * ev_o}
         out["variants"][other["kernel"]] = other
     if not args.plain_only:
-        runs = {"batched": run_batched, "adjoint": run_adjoint,
-                "adjoint_batched": run_adjoint_batched}
+        runs = {"host_api": run_host_api, "cycle": run_cycle, "batched": run_batched,
+                "adjoint": run_adjoint, "adjoint_batched": run_adjoint_batched}
         for name, fn in runs.items():
             if args.skip and name in args.skip.split(","):
                 continue

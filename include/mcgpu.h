@@ -31,6 +31,9 @@
  *  - Functions taking a `void *stream` accept a hipStream_t; NULL is HIP's
  *    default (null) stream.  mcgpu_simulate (host buffers) uses the
  *    context's own stream.  No torch or HIP types appear in signatures.
+ *  - Every call returns after its work has finished.  A context (and a
+ *    batch plan) holds scratch buffers reused by its calls: use it from one
+ *    thread at a time; separate contexts are independent.
  */
 #ifndef MCGPU_H
 #define MCGPU_H
