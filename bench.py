@@ -436,7 +436,12 @@ def main():
                    "kernel": args.kernel, "events_per_step": ev_step},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None if traffic is None else traffic * ev_step},
+                     "traffic": None if traffic is None else traffic * ev_step,
+                     "note": ("72 algorithmic B/event (36-byte state read + written, the "
+                              "event formulation); the history kernel keeps the state in "
+                              "registers (measured 0.06 B/event) and is bound by FP64 VALU "
+                              "issue, hence frac > 1") if args.kernel == "history" else
+                             "72 algorithmic B/event; measured 88 B/event (44-byte pool slots)"},
         "cpu_baseline": None,
         "variants": {},
     }

@@ -178,7 +178,9 @@ typedef struct mcgpu_batch mcgpu_batch;
 /* Build a plan: copies the slab table to the device, orders the tiles
    (largest work first) and captures setup + transport + finalize into a
    hipGraph.  opt: block_size, grid_blocks, events_per_refill and
-   particles_per_lane (tile = particles_per_lane * block_size particles). */
+   particles_per_lane (tile = particles_per_lane * block_size particles).
+   Slabs of any cell count: a tile keeps its slab's tally (and cross
+   sections) in LDS when they fit and adds to the global tally otherwise. */
 int mcgpu_batch_create(mcgpu_context *ctx, int32_t n_slabs, const mcgpu_slab *slabs,
                        const double *sigs, const double *absorption_rates,
                        double *weights_absorbed, int64_t total_cells, const mcgpu_bank *bank,

@@ -64,6 +64,92 @@ __device__ __forceinline__ int next_tile(mc_counters *ctr, int *s_tile) {
   return *s_tile;
 }
 
+/* Per-lane counters of one block, carried across its tiles. */
+struct blane {
+  bool capped;
+  unsigned long long ev, col;
+};
+
+/* One tile of the forward: its histories, lanes refilled from the LDS queue
+   s_next.  Called with LDS or global accumulator/cell pointers from two
+   places, so each inlined copy uses the matching memory instructions. */
+template <bool SCORE>
+__device__ __forceinline__ void fwd_tile(const bcell_args &a, const btile_k &tile,
+                                         const bslab_k &sl, const mc_layer_k &L,
+                                         unsigned long long *t_lo, unsigned long long *t_hi,
+                                         const mc_cell *cells, const double *g,
+                                         unsigned long long *s_next, unsigned long long *s_tot,
+                                         uint32_t max_ev, int refill_k, blane &bl) {
+  const int lane = __lane_id();
+  mc_state p;
+  p.index = 0;
+  int64_t idx = 0;
+  bool has = false, done = false;
+  uint32_t ev = 0, col = 0;
+  double score = 0.0;
+  for (;;) {
+    if (!done) {
+      const unsigned long long need = __ballot(!has);
+      if (need) {
+        const int cnt = __popcll(need);
+        const int leader = __ffsll((long long)need) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(s_next, (unsigned long long)cnt);
+        base = mc_readlane64(base, leader);
+        if (!has) {
+          const int64_t i = (int64_t)(base + (unsigned long long)mc_mbcnt64(need));
+          if (i < tile.end) {
+            p.seed = a.bank.seed[i];
+            p.x = a.bank.x[i];
+            p.mu = a.bank.mu[i];
+            p.wmc = a.bank.wmc[i];
+            p.index = a.bank.index[i];
+            idx = i;
+            has = true;
+            ev = 0;
+            col = 0;
+            score = 0.0;
+          }
+        }
+        if (base + (unsigned long long)cnt >= (unsigned long long)tile.end) done = true;
+      }
+    }
+    if (__ballot(has) == 0) break;
+    if (has) {
+      for (int k = 0; k < refill_k; ++k) {
+        if (!mc_alive(p, L) || ev >= max_ev) break;
+        const int c = p.index - L.index_start;
+        bool coll;
+        double di;
+        const double dw = mc_event(p, cells[c], L.dx, coll, di);
+        ++ev;
+        col += coll ? 1u : 0u;
+        mc_tally(t_lo, t_hi, a.flags + sl.cell_off, &a.ctr->status, c, dw);
+        if (SCORE) score = score + g[c] * dw;
+      }
+      const bool alive = mc_alive(p, L);
+      if (!alive || ev >= max_ev) {
+        int code = mc_exit_code(p, L);
+        if (alive) {
+          bl.capped = true;
+          code = 2;
+        }
+        a.bank.seed[idx] = p.seed;
+        a.bank.x[idx] = p.x;
+        a.bank.mu[idx] = p.mu;
+        a.bank.wmc[idx] = p.wmc;
+        a.bank.index[idx] = p.index;
+        if (a.result8) a.result8[idx] = (int8_t)code;
+        if (SCORE) a.score_out[idx] = score;
+        bl.ev += ev;
+        bl.col += col;
+        atomicAdd(&s_tot[code == -1 ? 2 : (code == 1 ? 3 : 4)], code == 2 ? 0ull : 1ull);
+        has = false;
+      }
+    }
+  }
+}
+
 /* Forward: mcgpu_forward_cells per slab (history kernel arithmetic). */
 template <bool SCORE>
 __global__ __launch_bounds__(1024) void k_forward_cells_batch(bcell_args a, int refill_k) {
@@ -74,9 +160,11 @@ __global__ __launch_bounds__(1024) void k_forward_cells_batch(bcell_args a, int 
   const int lane = __lane_id();
   const int B = blockDim.x, tid = threadIdx.x;
   if (tid < 5) s_tot[tid] = 0;
-  bool capped = false;
-  unsigned long long lane_ev = 0, lane_col = 0;
+  blane bl;
+  bl.capped = false;
+  bl.ev = bl.col = 0;
   const uint32_t max_ev = a.max_events > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)a.max_events;
+  unsigned long long *t_lds = reinterpret_cast<unsigned long long *>(smem);
 
   for (;;) {
     const int t = next_tile(a.ctr, &s_tile);
@@ -90,104 +178,33 @@ __global__ __launch_bounds__(1024) void k_forward_cells_batch(bcell_args a, int 
     L.dx = sl.dx;
     L.min_weight = sl.min_weight;
     const bool in_lds = m <= a.lds_max_m; /* block-uniform */
-    unsigned long long *t_lo, *t_hi;
-    const mc_cell *cells;
-    const double *g;
+    mc_cell *lc = reinterpret_cast<mc_cell *>(smem + (size_t)16 * m);
+    double *lg = reinterpret_cast<double *>(smem + (size_t)32 * m);
     if (in_lds) {
-      t_lo = reinterpret_cast<unsigned long long *>(smem);
-      t_hi = t_lo + m;
-      mc_cell *lc = reinterpret_cast<mc_cell *>(smem + (size_t)16 * m);
-      double *lg = reinterpret_cast<double *>(smem + (size_t)32 * m);
       for (int c = tid; c < m; c += B) {
-        t_lo[c] = 0;
-        t_hi[c] = 0;
+        t_lds[c] = 0;
+        t_lds[m + c] = 0;
         lc[c] = a.cells[sl.cell_off + c];
         if (SCORE) lg[c] = a.g[sl.cell_off + c];
       }
-      cells = lc;
-      g = lg;
-    } else {
-      t_lo = a.fx + sl.cell_off;
-      t_hi = a.fx + a.total + sl.cell_off;
-      cells = a.cells + sl.cell_off;
-      g = a.g ? a.g + sl.cell_off : nullptr;
     }
     if (tid == 0) s_next = (unsigned long long)tile.begin;
     __syncthreads();
 
-    mc_state p;
-    p.index = 0;
-    int64_t idx = 0;
-    bool has = false, done = false;
-    uint32_t ev = 0, col = 0;
-    double score = 0.0;
-    for (;;) {
-      if (!done) {
-        const unsigned long long need = __ballot(!has);
-        if (need) {
-          const int cnt = __popcll(need);
-          const int leader = __ffsll((long long)need) - 1;
-          unsigned long long base = 0;
-          if (lane == leader) base = atomicAdd(&s_next, (unsigned long long)cnt);
-          base = mc_readlane64(base, leader);
-          if (!has) {
-            const int64_t i = (int64_t)(base + (unsigned long long)mc_mbcnt64(need));
-            if (i < tile.end) {
-              p.seed = a.bank.seed[i];
-              p.x = a.bank.x[i];
-              p.mu = a.bank.mu[i];
-              p.wmc = a.bank.wmc[i];
-              p.index = a.bank.index[i];
-              idx = i;
-              has = true;
-              ev = 0;
-              col = 0;
-              score = 0.0;
-            }
-          }
-          if (base + (unsigned long long)cnt >= (unsigned long long)tile.end) done = true;
-        }
-      }
-      if (__ballot(has) == 0) break;
-      if (has) {
-        for (int k = 0; k < refill_k; ++k) {
-          if (!mc_alive(p, L) || ev >= max_ev) break;
-          const int c = p.index - L.index_start;
-          bool coll;
-          double di;
-          const double dw = mc_event(p, cells[c], L.dx, coll, di);
-          ++ev;
-          col += coll ? 1u : 0u;
-          mc_tally(t_lo, t_hi, a.flags + sl.cell_off, &a.ctr->status, c, dw);
-          if (SCORE) score = score + g[c] * dw;
-        }
-        const bool alive = mc_alive(p, L);
-        if (!alive || ev >= max_ev) {
-          int code = mc_exit_code(p, L);
-          if (alive) {
-            capped = true;
-            code = 2;
-          }
-          a.bank.seed[idx] = p.seed;
-          a.bank.x[idx] = p.x;
-          a.bank.mu[idx] = p.mu;
-          a.bank.wmc[idx] = p.wmc;
-          a.bank.index[idx] = p.index;
-          if (a.result8) a.result8[idx] = (int8_t)code;
-          if (SCORE) a.score_out[idx] = score;
-          lane_ev += ev;
-          lane_col += col;
-          atomicAdd(&s_tot[code == -1 ? 2 : (code == 1 ? 3 : 4)], code == 2 ? 0ull : 1ull);
-          has = false;
-        }
-      }
-    }
+    if (in_lds)
+      fwd_tile<SCORE>(a, tile, sl, L, t_lds, t_lds + m, lc, lg, &s_next, s_tot, max_ev,
+                      refill_k, bl);
+    else
+      fwd_tile<SCORE>(a, tile, sl, L, a.fx + sl.cell_off, a.fx + a.total + sl.cell_off,
+                      a.cells + sl.cell_off, a.g ? a.g + sl.cell_off : nullptr, &s_next, s_tot,
+                      max_ev, refill_k, bl);
+
     __syncthreads();
     if (in_lds) {
       for (int c = tid; c < m; c += B) {
         mc_fx v;
-        v.lo = t_lo[c];
-        v.hi = t_hi[c];
+        v.lo = t_lds[c];
+        v.hi = t_lds[m + c];
         mc_fx_atomic_add(&a.fx[sl.cell_off + c], &a.fx[a.total + sl.cell_off + c], v);
       }
     }
@@ -195,13 +212,13 @@ __global__ __launch_bounds__(1024) void k_forward_cells_batch(bcell_args a, int 
   }
 
   {
-    const unsigned long long e = mc_wave_sum(lane_ev), cl = mc_wave_sum(lane_col);
+    const unsigned long long e = mc_wave_sum(bl.ev), cl = mc_wave_sum(bl.col);
     if (lane == 0) {
       atomicAdd(&s_tot[0], e);
       atomicAdd(&s_tot[1], cl);
     }
   }
-  if (capped) atomicOr(&a.ctr->status, 1u);
+  if (bl.capped) atomicOr(&a.ctr->status, 1u);
   __syncthreads();
   if (tid == 0) {
     atomicAdd(&a.ctr->events, s_tot[0]);
@@ -209,6 +226,100 @@ __global__ __launch_bounds__(1024) void k_forward_cells_batch(bcell_args a, int 
     atomicAdd(&a.ctr->n_left, s_tot[2]);
     atomicAdd(&a.ctr->n_right, s_tot[3]);
     atomicAdd(&a.ctr->n_killed, s_tot[4]);
+  }
+}
+
+/* One tile of the adjoint (replay from the initial bank); pointers as in
+   fwd_tile. */
+template <bool HAVE_SCORE>
+__device__ __forceinline__ void adj_tile(const bcell_args &a, const btile_k &tile,
+                                         const bslab_k &sl, const mc_layer_k &L,
+                                         unsigned long long *ga_lo, unsigned long long *ga_hi,
+                                         unsigned long long *gi_lo, unsigned long long *gi_hi,
+                                         const mc_cell *cells, const double *g,
+                                         unsigned long long *s_next, uint32_t max_ev,
+                                         int refill_k, blane &bl) {
+  const int lane = __lane_id();
+  uint32_t *fa = a.flags + sl.cell_off, *fi = a.flags + a.total + sl.cell_off;
+  mc_state p;
+  p.index = 0;
+  int64_t idx = 0;
+  bool has = false, done = false;
+  int phase = 1; /* 0: score replay, 1: gradient replay */
+  uint32_t ev = 0;
+  double T = 0.0, P = 0.0;
+  for (;;) {
+    if (!done) {
+      const unsigned long long need = __ballot(!has);
+      if (need) {
+        const int cnt = __popcll(need);
+        const int leader = __ffsll((long long)need) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(s_next, (unsigned long long)cnt);
+        base = mc_readlane64(base, leader);
+        if (!has) {
+          const int64_t i = (int64_t)(base + (unsigned long long)mc_mbcnt64(need));
+          if (i < tile.end) {
+            p.seed = a.bank.seed[i];
+            p.x = a.bank.x[i];
+            p.mu = a.bank.mu[i];
+            p.wmc = a.bank.wmc[i];
+            p.index = a.bank.index[i];
+            idx = i;
+            has = true;
+            ev = 0;
+            P = 0.0;
+            if (HAVE_SCORE) {
+              T = a.score_in[i];
+              phase = 1;
+            } else {
+              T = 0.0;
+              phase = 0;
+            }
+          }
+        }
+        if (base + (unsigned long long)cnt >= (unsigned long long)tile.end) done = true;
+      }
+    }
+    if (__ballot(has) == 0) break;
+    if (has) {
+      for (int k = 0; k < refill_k; ++k) {
+        if (!mc_alive(p, L) || ev >= max_ev) break;
+        const int c = p.index - L.index_start;
+        const mc_cell cell = cells[c];
+        bool coll;
+        double di;
+        const double dw = mc_event(p, cell, L.dx, coll, di);
+        ++ev;
+        if (!HAVE_SCORE && phase == 0) {
+          T = T + g[c] * dw;
+          continue;
+        }
+        const double gc = g[c];
+        double term_a, term_i;
+        const bool has_i = mc_adjoint_terms(cell, coll, di, dw, gc, p.wmc, T, P, term_a, term_i);
+        mc_tally<true>(ga_lo, ga_hi, fa, &a.ctr->status, c, term_a);
+        if (has_i) mc_tally<true>(gi_lo, gi_hi, fi, &a.ctr->status, c, term_i);
+        P = P + gc * dw;
+      }
+      const bool alive = mc_alive(p, L);
+      if (!alive || ev >= max_ev) {
+        if (alive) bl.capped = true;
+        bl.ev += ev;
+        if (!HAVE_SCORE && phase == 0 && !alive) {
+          p.seed = a.bank.seed[idx];
+          p.x = a.bank.x[idx];
+          p.mu = a.bank.mu[idx];
+          p.wmc = a.bank.wmc[idx];
+          p.index = a.bank.index[idx];
+          ev = 0;
+          P = 0.0;
+          phase = 1;
+        } else {
+          has = false;
+        }
+      }
+    }
   }
 }
 
@@ -220,10 +331,12 @@ __global__ __launch_bounds__(1024) void k_adjoint_cells_batch(bcell_args a, int 
   __shared__ unsigned long long s_next;
   const int lane = __lane_id();
   const int B = blockDim.x, tid = threadIdx.x;
-  bool capped = false;
-  unsigned long long lane_ev = 0;
+  blane bl;
+  bl.capped = false;
+  bl.ev = bl.col = 0;
   const uint32_t max_ev = a.max_events > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)a.max_events;
   const int64_t T4 = a.total;
+  unsigned long long *w_lds = reinterpret_cast<unsigned long long *>(smem);
 
   for (;;) {
     const int t = next_tile(a.ctr, &s_tile);
@@ -237,135 +350,47 @@ __global__ __launch_bounds__(1024) void k_adjoint_cells_batch(bcell_args a, int 
     L.dx = sl.dx;
     L.min_weight = sl.min_weight;
     const bool in_lds = m <= a.lds_max_m; /* block-uniform */
-    unsigned long long *ga_lo, *ga_hi, *gi_lo, *gi_hi;
-    const mc_cell *cells;
-    const double *g;
+    mc_cell *lc = reinterpret_cast<mc_cell *>(smem + (size_t)32 * m);
+    double *lg = reinterpret_cast<double *>(smem + (size_t)48 * m);
     if (in_lds) {
-      ga_lo = reinterpret_cast<unsigned long long *>(smem);
-      ga_hi = ga_lo + m;
-      gi_lo = ga_lo + 2 * m;
-      gi_hi = ga_lo + 3 * m;
-      mc_cell *lc = reinterpret_cast<mc_cell *>(smem + (size_t)32 * m);
-      double *lg = reinterpret_cast<double *>(smem + (size_t)48 * m);
       for (int c = tid; c < m; c += B) {
-        ga_lo[c] = 0;
-        ga_hi[c] = 0;
-        gi_lo[c] = 0;
-        gi_hi[c] = 0;
+        w_lds[c] = 0;
+        w_lds[m + c] = 0;
+        w_lds[2 * m + c] = 0;
+        w_lds[3 * m + c] = 0;
         lc[c] = a.cells[sl.cell_off + c];
         lg[c] = a.g[sl.cell_off + c];
       }
-      cells = lc;
-      g = lg;
-    } else {
-      ga_lo = a.fx + sl.cell_off;
-      ga_hi = a.fx + T4 + sl.cell_off;
-      gi_lo = a.fx + 2 * T4 + sl.cell_off;
-      gi_hi = a.fx + 3 * T4 + sl.cell_off;
-      cells = a.cells + sl.cell_off;
-      g = a.g + sl.cell_off;
     }
-    uint32_t *fa = a.flags + sl.cell_off, *fi = a.flags + T4 + sl.cell_off;
     if (tid == 0) s_next = (unsigned long long)tile.begin;
     __syncthreads();
 
-    mc_state p;
-    p.index = 0;
-    int64_t idx = 0;
-    bool has = false, done = false;
-    int phase = 1; /* 0: score replay, 1: gradient replay */
-    uint32_t ev = 0;
-    double T = 0.0, P = 0.0;
-    for (;;) {
-      if (!done) {
-        const unsigned long long need = __ballot(!has);
-        if (need) {
-          const int cnt = __popcll(need);
-          const int leader = __ffsll((long long)need) - 1;
-          unsigned long long base = 0;
-          if (lane == leader) base = atomicAdd(&s_next, (unsigned long long)cnt);
-          base = mc_readlane64(base, leader);
-          if (!has) {
-            const int64_t i = (int64_t)(base + (unsigned long long)mc_mbcnt64(need));
-            if (i < tile.end) {
-              p.seed = a.bank.seed[i];
-              p.x = a.bank.x[i];
-              p.mu = a.bank.mu[i];
-              p.wmc = a.bank.wmc[i];
-              p.index = a.bank.index[i];
-              idx = i;
-              has = true;
-              ev = 0;
-              P = 0.0;
-              if (HAVE_SCORE) {
-                T = a.score_in[i];
-                phase = 1;
-              } else {
-                T = 0.0;
-                phase = 0;
-              }
-            }
-          }
-          if (base + (unsigned long long)cnt >= (unsigned long long)tile.end) done = true;
-        }
-      }
-      if (__ballot(has) == 0) break;
-      if (has) {
-        for (int k = 0; k < refill_k; ++k) {
-          if (!mc_alive(p, L) || ev >= max_ev) break;
-          const int c = p.index - L.index_start;
-          const mc_cell cell = cells[c];
-          bool coll;
-          double di;
-          const double dw = mc_event(p, cell, L.dx, coll, di);
-          ++ev;
-          if (!HAVE_SCORE && phase == 0) {
-            T = T + g[c] * dw;
-            continue;
-          }
-          const double gc = g[c];
-          double term_a, term_i;
-          const bool has_i = mc_adjoint_terms(cell, coll, di, dw, gc, p.wmc, T, P, term_a, term_i);
-          mc_tally<true>(ga_lo, ga_hi, fa, &a.ctr->status, c, term_a);
-          if (has_i) mc_tally<true>(gi_lo, gi_hi, fi, &a.ctr->status, c, term_i);
-          P = P + gc * dw;
-        }
-        const bool alive = mc_alive(p, L);
-        if (!alive || ev >= max_ev) {
-          if (alive) capped = true;
-          lane_ev += ev;
-          if (!HAVE_SCORE && phase == 0 && !alive) {
-            p.seed = a.bank.seed[idx];
-            p.x = a.bank.x[idx];
-            p.mu = a.bank.mu[idx];
-            p.wmc = a.bank.wmc[idx];
-            p.index = a.bank.index[idx];
-            ev = 0;
-            P = 0.0;
-            phase = 1;
-          } else {
-            has = false;
-          }
-        }
-      }
-    }
+    if (in_lds)
+      adj_tile<HAVE_SCORE>(a, tile, sl, L, w_lds, w_lds + m, w_lds + 2 * m, w_lds + 3 * m, lc, lg,
+                           &s_next, max_ev, refill_k, bl);
+    else
+      adj_tile<HAVE_SCORE>(a, tile, sl, L, a.fx + sl.cell_off, a.fx + T4 + sl.cell_off,
+                           a.fx + 2 * T4 + sl.cell_off, a.fx + 3 * T4 + sl.cell_off,
+                           a.cells + sl.cell_off, a.g + sl.cell_off, &s_next, max_ev, refill_k,
+                           bl);
+
     __syncthreads();
     if (in_lds) {
       for (int c = tid; c < m; c += B) {
         mc_fx v;
-        v.lo = ga_lo[c];
-        v.hi = ga_hi[c];
+        v.lo = w_lds[c];
+        v.hi = w_lds[m + c];
         mc_fx_atomic_add(&a.fx[sl.cell_off + c], &a.fx[T4 + sl.cell_off + c], v);
-        v.lo = gi_lo[c];
-        v.hi = gi_hi[c];
+        v.lo = w_lds[2 * m + c];
+        v.hi = w_lds[3 * m + c];
         mc_fx_atomic_add(&a.fx[2 * T4 + sl.cell_off + c], &a.fx[3 * T4 + sl.cell_off + c], v);
       }
     }
     __syncthreads();
   }
-  lane_ev = mc_wave_sum(lane_ev);
-  if (lane == 0) atomicAdd(&a.ctr->events, lane_ev);
-  if (capped) atomicOr(&a.ctr->status, 1u);
+  const unsigned long long e = mc_wave_sum(bl.ev);
+  if (lane == 0) atomicAdd(&a.ctr->events, e);
+  if (bl.capped) atomicOr(&a.ctr->status, 1u);
 }
 
 /* cells from (sig_a, sig_i); zeroed accumulators (words per cell) and counters */

@@ -45,6 +45,8 @@ struct batch_args {
   const tile_k *tiles;
   int32_t n_tiles;
   int32_t lds_cells_max_m; /* slabs with m <= this stage cross sections in LDS */
+  int32_t lds_tally_max_m; /* slabs with m <= this keep their tally in LDS; larger
+                              ones add straight into the global tally */
   mcgpu_bank soa;
   const mc_cell *cells;
   unsigned long long *fx; /* lo words [0, total), hi words [total, 2 total) */
@@ -55,18 +57,106 @@ struct batch_args {
   uint64_t max_events;
 };
 
+/* Per-lane counters of one block, carried across its tiles. */
+struct batch_lane {
+  bool capped;
+  unsigned int ev_pending, col_pending;
+};
+
+/* One tile: its histories, lanes refilled from the LDS queue s_next.  The
+   tally and cell pointers are LDS or global according to the template
+   arguments, so every access compiles to the matching instruction. */
+template <bool LDS_TALLY, bool LDS_CELLS>
+__device__ __forceinline__ void batch_tile(const batch_args &a, const tile_k &tile,
+                                           const slab_k &sl, const mc_layer_k &L,
+                                           unsigned long long *t_lo, unsigned long long *t_hi,
+                                           const mc_cell *cells, unsigned long long *s_next,
+                                           unsigned long long *s_tot, uint32_t max_ev,
+                                           int refill_k, batch_lane &bl) {
+  const int lane = __lane_id();
+  mc_state p;
+  p.index = 0;
+  int64_t idx = 0;
+  bool has = false, done = false;
+  uint32_t ev = 0, col = 0;
+  for (;;) {
+    if (!done) {
+      const unsigned long long need = __ballot(!has);
+      if (need) {
+        const int cnt = __popcll(need);
+        const int leader = __ffsll((long long)need) - 1;
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(s_next, (unsigned long long)cnt);
+        base = mc_readlane64(base, leader);
+        if (!has) {
+          const int64_t i = (int64_t)(base + (unsigned long long)mc_mbcnt64(need));
+          if (i < tile.end) {
+            p.seed = a.soa.seed[i];
+            p.x = a.soa.x[i];
+            p.mu = a.soa.mu[i];
+            p.wmc = a.soa.wmc[i];
+            p.index = a.soa.index[i];
+            idx = i;
+            has = true;
+            ev = 0;
+            col = 0;
+          }
+        }
+        if (base + (unsigned long long)cnt >= (unsigned long long)tile.end) done = true;
+      }
+    }
+    if (__ballot(has) == 0) break;
+    if (has) {
+      for (int k = 0; k < refill_k; ++k) {
+        if (!mc_alive(p, L) || ev >= max_ev) break;
+        const int c = p.index - L.index_start;
+        bool coll;
+        double di;
+        const double dw = mc_event(p, cells[c], L.dx, coll, di);
+        ++ev;
+        col += coll ? 1u : 0u;
+        mc_tally(t_lo, t_hi, a.flags + sl.cell_off, &a.ctr->status, c, dw);
+      }
+      const bool alive = mc_alive(p, L);
+      if (!alive || ev >= max_ev) {
+        int code = mc_exit_code(p, L);
+        if (alive) {
+          bl.capped = true;
+          code = 2;
+        }
+        a.soa.seed[idx] = p.seed;
+        a.soa.x[idx] = p.x;
+        a.soa.mu[idx] = p.mu;
+        a.soa.wmc[idx] = p.wmc;
+        a.soa.index[idx] = p.index;
+        if (a.result8) a.result8[idx] = (int8_t)code;
+        bl.ev_pending += ev;
+        bl.col_pending += col;
+        atomicAdd(&s_tot[code == -1 ? 2 : (code == 1 ? 3 : 4)], code == 2 ? 0ull : 1ull);
+        has = false;
+        if (bl.ev_pending > 0x40000000u) { /* keep the 32-bit lane counters safe */
+          atomicAdd(&s_tot[0], (unsigned long long)bl.ev_pending);
+          atomicAdd(&s_tot[1], (unsigned long long)bl.col_pending);
+          bl.ev_pending = bl.col_pending = 0;
+        }
+      }
+    }
+  }
+}
+
 __global__ __launch_bounds__(1024) void k_batched(batch_args a, int refill_k) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_tile;
   __shared__ unsigned long long s_next;
   __shared__ unsigned long long s_tot[5];
-  unsigned long long *t_lo = reinterpret_cast<unsigned long long *>(smem);
+  unsigned long long *t_lds = reinterpret_cast<unsigned long long *>(smem);
   const int lane = __lane_id();
   const int B = blockDim.x;
   const int tid = threadIdx.x;
   if (tid < 5) s_tot[tid] = 0;
-  bool capped = false;
-  unsigned int lane_ev_pending = 0, lane_col_pending = 0;
+  batch_lane bl;
+  bl.capped = false;
+  bl.ev_pending = bl.col_pending = 0;
   const uint32_t max_ev = a.max_events > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)a.max_events;
 
   for (;;) {
@@ -82,105 +172,50 @@ __global__ __launch_bounds__(1024) void k_batched(batch_args a, int refill_k) {
     L.m = m;
     L.dx = sl.dx;
     L.min_weight = sl.min_weight;
-    unsigned long long *t_hi = t_lo + m;
-    const bool lds_cells = m <= a.lds_cells_max_m;
+    const bool lds_tally = m <= a.lds_tally_max_m; /* block-uniform */
+    const bool lds_cells = m <= a.lds_cells_max_m; /* implies lds_tally */
     mc_cell *lc = reinterpret_cast<mc_cell *>(smem + (size_t)16 * m);
-    const mc_cell *cells = lds_cells ? lc : a.cells + sl.cell_off;
-    for (int c = tid; c < m; c += B) {
-      t_lo[c] = 0;
-      t_hi[c] = 0;
-      if (lds_cells) lc[c] = a.cells[sl.cell_off + c];
+    if (lds_tally) {
+      for (int c = tid; c < m; c += B) {
+        t_lds[c] = 0;
+        t_lds[m + c] = 0;
+        if (lds_cells) lc[c] = a.cells[sl.cell_off + c];
+      }
     }
     if (tid == 0) s_next = (unsigned long long)tile.begin;
     __syncthreads();
 
-    /* the tile's histories, lanes refilled from the LDS queue */
-    mc_state p;
-    p.index = 0;
-    int64_t idx = 0;
-    bool has = false, done = false;
-    uint32_t ev = 0, col = 0;
-    for (;;) {
-      if (!done) {
-        const unsigned long long need = __ballot(!has);
-        if (need) {
-          const int cnt = __popcll(need);
-          const int leader = __ffsll((long long)need) - 1;
-          unsigned long long base = 0;
-          if (lane == leader) base = atomicAdd(&s_next, (unsigned long long)cnt);
-          base = mc_readlane64(base, leader);
-          if (!has) {
-            const int64_t i = (int64_t)(base + (unsigned long long)mc_mbcnt64(need));
-            if (i < tile.end) {
-              p.seed = a.soa.seed[i];
-              p.x = a.soa.x[i];
-              p.mu = a.soa.mu[i];
-              p.wmc = a.soa.wmc[i];
-              p.index = a.soa.index[i];
-              idx = i;
-              has = true;
-              ev = 0;
-              col = 0;
-            }
-          }
-          if (base + (unsigned long long)cnt >= (unsigned long long)tile.end) done = true;
-        }
-      }
-      if (__ballot(has) == 0) break;
-      if (has) {
-        for (int k = 0; k < refill_k; ++k) {
-          if (!mc_alive(p, L) || ev >= max_ev) break;
-          const int c = p.index - L.index_start;
-          bool coll;
-          double di;
-          const double dw = mc_event(p, cells[c], L.dx, coll, di);
-          ++ev;
-          col += coll ? 1u : 0u;
-          mc_tally(t_lo, t_hi, a.flags + sl.cell_off, &a.ctr->status, c, dw);
-        }
-        const bool alive = mc_alive(p, L);
-        if (!alive || ev >= max_ev) {
-          int code = mc_exit_code(p, L);
-          if (alive) {
-            capped = true;
-            code = 2;
-          }
-          a.soa.seed[idx] = p.seed;
-          a.soa.x[idx] = p.x;
-          a.soa.mu[idx] = p.mu;
-          a.soa.wmc[idx] = p.wmc;
-          a.soa.index[idx] = p.index;
-          if (a.result8) a.result8[idx] = (int8_t)code;
-          lane_ev_pending += ev;
-          lane_col_pending += col;
-          atomicAdd(&s_tot[code == -1 ? 2 : (code == 1 ? 3 : 4)], code == 2 ? 0ull : 1ull);
-          has = false;
-          if (lane_ev_pending > 0x40000000u) { /* keep the 32-bit lane counters safe */
-            atomicAdd(&s_tot[0], (unsigned long long)lane_ev_pending);
-            atomicAdd(&s_tot[1], (unsigned long long)lane_col_pending);
-            lane_ev_pending = lane_col_pending = 0;
-          }
-        }
-      }
-    }
+    if (lds_cells)
+      batch_tile<true, true>(a, tile, sl, L, t_lds, t_lds + m, lc, &s_next, s_tot, max_ev,
+                             refill_k, bl);
+    else if (lds_tally)
+      batch_tile<true, false>(a, tile, sl, L, t_lds, t_lds + m, a.cells + sl.cell_off, &s_next,
+                              s_tot, max_ev, refill_k, bl);
+    else
+      batch_tile<false, false>(a, tile, sl, L, a.fx + sl.cell_off,
+                               a.fx + a.total_cells + sl.cell_off, a.cells + sl.cell_off,
+                               &s_next, s_tot, max_ev, refill_k, bl);
+
     __syncthreads();
-    for (int c = tid; c < m; c += B) {
-      mc_fx v;
-      v.lo = t_lo[c];
-      v.hi = t_hi[c];
-      mc_fx_atomic_add(&a.fx[sl.cell_off + c], &a.fx[a.total_cells + sl.cell_off + c], v);
+    if (lds_tally) {
+      for (int c = tid; c < m; c += B) {
+        mc_fx v;
+        v.lo = t_lds[c];
+        v.hi = t_lds[m + c];
+        mc_fx_atomic_add(&a.fx[sl.cell_off + c], &a.fx[a.total_cells + sl.cell_off + c], v);
+      }
     }
     __syncthreads();
   }
 
   {
-    const unsigned long long e = mc_wave_sum(lane_ev_pending), cl = mc_wave_sum(lane_col_pending);
+    const unsigned long long e = mc_wave_sum(bl.ev_pending), cl = mc_wave_sum(bl.col_pending);
     if (lane == 0) {
       atomicAdd(&s_tot[0], e);
       atomicAdd(&s_tot[1], cl);
     }
   }
-  if (capped) atomicOr(&a.ctr->status, 1u);
+  if (bl.capped) atomicOr(&a.ctr->status, 1u);
   __syncthreads();
   if (tid == 0) {
     atomicAdd(&a.ctr->events, s_tot[0]);
@@ -242,6 +277,7 @@ struct mcgpu_batch {
   int max_m = 0;
   size_t lds = 0;
   int lds_cells_max_m = 0;
+  int lds_tally_max_m = 0;
   int block = 1024, grid = 0, refill = 8;
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -296,12 +332,13 @@ int batch_capture(mcgpu_batch *b) {
   b->refill = b->opt.events_per_refill > 0 ? b->opt.events_per_refill : 8;
   int rc = batch_tiles(b);
   if (rc) return rc;
-  /* LDS: the largest slab's tally, plus its cross sections if they fit */
-  const size_t tally = (size_t)16 * b->max_m;
-  if (tally + 128 > ctx->lds_per_block) /* 128 B left for the kernel's static LDS */
-    return mc_fail(ctx, MCGPU_ERR_ARG, "batched: a slab's tally exceeds the LDS of one block");
-  b->lds = std::min(ctx->lds_per_block - 128, (size_t)32 * b->max_m);
+  /* LDS: the largest slab's tally plus its cross sections, as far as they fit
+     (128 B are left for the kernel's static LDS); slabs whose tally does not
+     fit add into the global tally directly */
+  const size_t budget = ctx->lds_per_block - 128;
+  b->lds = std::min(budget, (size_t)32 * b->max_m);
   b->lds_cells_max_m = (int)(b->lds / 32);
+  b->lds_tally_max_m = (int)(b->lds / 16);
   if (b->lds > 65536)
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_batched),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds);
@@ -327,6 +364,7 @@ int batch_capture(mcgpu_batch *b) {
   a.tiles = b->d_tiles;
   a.n_tiles = b->n_tiles;
   a.lds_cells_max_m = b->lds_cells_max_m;
+  a.lds_tally_max_m = b->lds_tally_max_m;
   a.soa = b->bank;
   a.cells = b->d_cells;
   a.fx = b->d_fx;

@@ -1,8 +1,8 @@
 /*
  * oracle.c -- TEST INFRASTRUCTURE ONLY.
  *
- * CPU oracle for mc-mpi's Layer::simulate: a line-for-line restatement of
- * src/layer.cpp and src/random.cpp (reference paths), built as liboracle.so and
+ * CPU oracle for mc-mpi's Layer::simulate: an operation-by-operation C
+ * restatement of src/layer.cpp and src/random.cpp (reference paths), built as liboracle.so and
  * used by tests/, __graft_entry__.smoke() and bench.py's CPU baseline to check
  * the HIP product.  The product never links, loads or calls it.
  *

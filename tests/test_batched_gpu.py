@@ -75,7 +75,8 @@ def _same_bank(a, b):
     return True
 
 
-EXTRA = ((8192, 3000), (5200, 2000), (40, 0), (33, 5))  # cells in L2 / LDS, empty slab
+# cells in L2 / LDS, empty slab, and a tally larger than LDS (added in global memory)
+EXTRA = ((8192, 3000), (5200, 2000), (40, 0), (33, 5), (12000, 1500))
 
 
 def test_batched_equals_unbatched_and_oracle(ctx, oracle):
