@@ -16,7 +16,9 @@
  *   - Taylor polynomials with exact rational coefficients 1/n and 1/n!;
  *   - double-double arithmetic (Dekker 1971; Knuth's TwoSum) with explicit
  *     fma, so no step depends on compiler contraction;
- *   - Ziv's strategy: a fast evaluation with relative error below 2^-72, a
+ *   - Ziv's strategy: a fast evaluation with relative error below 2^-72 (an
+ *     fma in place of a multiply and an add only removes a rounding, so it
+ *     never loosens that bound), a
  *     rounding test against a 2^-67 bound, and, when the test cannot decide, an
  *     all-double-double evaluation (relative error about 2^-100).  The result
  *     can differ from the correctly rounded one only when the exact value lies
@@ -182,11 +184,11 @@ MC_FN double mc_log(double x) {
   double ed = (double)e;
   double s2, serr2, s3, serr3;
   const double s = ed * MC_LN2_HI + mc_log_tab[idx][1];
-  double lo = ed * MC_LN2_MID + mc_log_tab[idx][2];
+  double lo = mc_fma(ed, MC_LN2_MID, mc_log_tab[idx][2]);
   mc_fast_two_sum(s, r, &s2, &serr2);
   lo = lo + serr2;
   mc_fast_two_sum(s2, -0.5 * r2h, &s3, &serr3);
-  lo = lo + (serr3 + (t - 0.5 * r2l));
+  lo = lo + (serr3 + mc_fma(-0.5, r2l, t));
   double hi;
   mc_fast_two_sum(s3, lo, &hi, &lo);
   if (mc_round_ok(hi, lo, __builtin_fabs(hi) * 0x1p-67)) return hi + lo;
@@ -296,11 +298,11 @@ MC_FN double mc_exp(double x) {
   if (__builtin_fabs(x) < 0x1p-54) return 1.0;
   double kd = __builtin_rint(x * MC_INV_LN2_128);
   int k = (int)kd;
-  double t = x - kd * MC_LN2_128_1; /* exact */
+  double t = mc_fma(-kd, MC_LN2_128_1, x); /* exact: kd*L1 is, and so is x - kd*L1 */
   double ph = kd * MC_LN2_128_2, pl = mc_fma(kd, MC_LN2_128_2, -ph);
   double rh, rl;
   mc_two_sum(t, -ph, &rh, &rl);
-  rl = rl - (pl + kd * MC_LN2_128_3);
+  rl = rl - mc_fma(kd, MC_LN2_128_3, pl);
   mc_fast_two_sum(rh, rl, &rh, &rl);
   /* exp(r) - 1 = r + r^2/2 + r^3 * Q(r), Q = 1/6 + r/24 + ... + r^4/5040 */
   double r2h = rh * rh;
@@ -312,7 +314,7 @@ MC_FN double mc_exp(double x) {
   q = (r2h * rh) * q;
   double ah, al;
   mc_fast_two_sum(rh, 0.5 * r2h, &ah, &al);
-  al = al + (rl + (0.5 * r2l + (rh * rl + q)));
+  al = al + (rl + mc_fma(0.5, r2l, mc_fma(rh, rl, q)));
   /* 1 + a */
   double eh, el;
   mc_fast_two_sum(1.0, ah, &eh, &el);
@@ -321,7 +323,7 @@ MC_FN double mc_exp(double x) {
   double th = mc_exp_tab[j][0], tl = mc_exp_tab[j][1];
   double yh = th * eh;
   double yl = mc_fma(th, eh, -yh);
-  yl = yl + (th * el + tl * eh);
+  yl = yl + mc_fma(th, el, tl * eh);
   mc_fast_two_sum(yh, yl, &yh, &yl);
   double sc = mc_pow2((k - j) / 128);
   if (mc_round_ok(yh, yl, yh * 0x1p-67)) return (yh + yl) * sc;
