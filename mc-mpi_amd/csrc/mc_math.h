@@ -18,8 +18,8 @@
  *     fma, so no step depends on compiler contraction;
  *   - Ziv's strategy: a fast evaluation with relative error below 2^-72 (an
  *     fma in place of a multiply and an add only removes a rounding, so it
- *     never loosens that bound), a
- *     rounding test against a 2^-67 bound, and, when the test cannot decide, an
+ *     never loosens that bound), a one-fma rounding test valid for errors
+ *     below 2^-68 (mc_rounds_to_hi), and, when the test cannot decide, an
  *     all-double-double evaluation (relative error about 2^-100).  The result
  *     can differ from the correctly rounded one only when the exact value lies
  *     within about 2^-100 (relative) of a rounding boundary.
@@ -94,6 +94,18 @@ MC_FN void mc_dd_add(double ah, double al, double bh, double bl, double *rh, dou
 MC_FN int mc_round_ok(double hi, double lo, double err) {
   return (hi + (lo + err) == hi) && (hi + (lo - err) == hi);
 }
+
+/* The same certificate in one fma, for hi + lo (hi = RN(hi + lo)) within
+   relative error eps of the exact value Y, eps < 2^-68 (the fast paths are
+   below 2^-72).  Let b be the distance from hi to its rounding boundary on
+   the side of lo: b >= 2^-54 |hi| (half an ulp of hi, or a quarter below a
+   power of two).  RN(hi + e lo) == hi with e = 1 + 2^-13 (one rounding, at
+   that boundary, ties included) gives |lo| <= b/e, so on lo's side
+   |Y - hi| <= b/e + eps |Y| < b, because b (1 - 1/e) > 2^-68 |hi|; on the
+   other side |Y - hi| <= eps |Y| is far below 2^-54 |hi|.  Y thus lies
+   strictly inside hi's rounding interval and rounds to hi. */
+#define MC_ZIV_E (1.0 + 0x1p-13)
+MC_FN int mc_rounds_to_hi(double hi, double lo) { return mc_fma(lo, MC_ZIV_E, hi) == hi; }
 
 /* ------------------------------------------------------------------- log */
 
@@ -191,7 +203,7 @@ MC_FN double mc_log(double x) {
   lo = lo + (serr3 + mc_fma(-0.5, r2l, t));
   double hi;
   mc_fast_two_sum(s3, lo, &hi, &lo);
-  if (mc_round_ok(hi, lo, __builtin_fabs(hi) * 0x1p-67)) return hi + lo;
+  if (mc_rounds_to_hi(hi, lo)) return hi;
   return mc_log_accurate(e, r, idx);
 }
 
@@ -326,7 +338,7 @@ MC_FN double mc_exp(double x) {
   yl = yl + mc_fma(th, el, tl * eh);
   mc_fast_two_sum(yh, yl, &yh, &yl);
   double sc = mc_pow2((k - j) / 128);
-  if (mc_round_ok(yh, yl, yh * 0x1p-67)) return (yh + yl) * sc;
+  if (mc_rounds_to_hi(yh, yl)) return yh * sc;
   if (__builtin_fabs(x) < 0x1p-30) return mc_exp_tiny(x);
   int kk;
   mc_exp_dd(x, 0.0, &yh, &yl, &kk);
