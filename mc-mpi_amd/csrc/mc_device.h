@@ -26,7 +26,10 @@
 
 __device__ __forceinline__ double mc_rnd_real(uint64_t &seed) {
   seed = (MC_RNG_G * seed + MC_RNG_C) & MC_MASK63;
-  return (double)seed * 0x1p-63; /* (double)seed rounds to nearest; scaling is exact */
+  /* (double)seed * 2^-63, the conversion rounded to nearest and the scaling
+     exact, formed as one rounding of hi*2^-31 + lo*2^-63 (both exact) */
+  const double hi = (double)(uint32_t)(seed >> 32), lo = (double)(uint32_t)seed;
+  return __builtin_fma(hi, 0x1p-31, lo * 0x1p-63);
 }
 
 __device__ __forceinline__ uint64_t mc_rnd_seed(uint64_t &seed) {
@@ -167,11 +170,16 @@ __device__ __forceinline__ unsigned long long mc_wave_sum(unsigned long long v) 
 template <bool SIGNED = false>
 __device__ __forceinline__ void mc_tally(unsigned long long *lo, unsigned long long *hi,
                                          uint32_t *flags, unsigned int *status, int c, double dw) {
-  uint32_t f = 0;
-  const mc_fx v = mc_fx_from_double_p(dw, &f, SIGNED ? 0 : 1);
-  if (f) {
-    atomicOr(&flags[c], f);
-    if (f & 8u) atomicOr(status, 2u);
+  mc_fx v;
+  int fast;
+  MC_FX_FAST(!SIGNED, dw, v, fast);
+  if (!fast) { /* flags can only come from the general conversion */
+    const mc_fx_flagged g = mc_fx_from_double_general(dw);
+    v = g.v;
+    if (g.flags) {
+      atomicOr(&flags[c], g.flags);
+      if (g.flags & 8u) atomicOr(status, 2u);
+    }
   }
   mc_fx_atomic_add(&lo[c], &hi[c], v);
 }

@@ -16,10 +16,11 @@
  *   - Taylor polynomials with exact rational coefficients 1/n and 1/n!;
  *   - double-double arithmetic (Dekker 1971; Knuth's TwoSum) with explicit
  *     fma, so no step depends on compiler contraction;
- *   - Ziv's strategy: a fast evaluation with relative error below 2^-72 (an
- *     fma in place of a multiply and an add only removes a rounding, so it
- *     never loosens that bound), a one-fma rounding test valid for errors
- *     below 2^-68 (mc_rounds_to_hi), and, when the test cannot decide, an
+ *   - Ziv's strategy: a fast evaluation with relative error below 2^-70
+ *     (arithmetic below 2^-72, series truncation below 2^-72; an fma in place
+ *     of a multiply and an add only removes a rounding), a one-fma rounding
+ *     test valid for errors below 2^-68 (mc_rounds_to_hi), and, when the test
+ *     cannot decide, an
  *     all-double-double evaluation (relative error about 2^-100).  The result
  *     can differ from the correctly rounded one only when the exact value lies
  *     within about 2^-100 (relative) of a rounding boundary.
@@ -178,10 +179,12 @@ MC_FN double mc_log(double x) {
   double c = mc_log_tab[idx][0];
   double r = mc_fma(m, c, -1.0); /* exact, |r| < 2^-8 */
 
-  /* log1p(r) = r - r^2/2 + r^3 * P(r), P = 1/3 - r/4 + ... + r^6/9 */
+  /* log1p(r) = r - r^2/2 + r^3 * P(r), P = 1/3 - r/4 + ... - r^5/8; the
+     omitted tail (r^9/9 - ...) is below 2^-74 of the result for |r| < 2^-8.4
+     (the largest |r| of the table; near x = 1, |r| < 2^-10 and log x ~ r) */
   double r2h = r * r;
   double r2l = mc_fma(r, r, -r2h);
-  double p = mc_fma(r, mc_fma(r, mc_inv_n[9][0], -mc_inv_n[8][0]), mc_inv_n[7][0]);
+  double p = mc_fma(r, -mc_inv_n[8][0], mc_inv_n[7][0]);
   p = mc_fma(r, p, -mc_inv_n[6][0]);
   p = mc_fma(r, p, mc_inv_n[5][0]);
   p = mc_fma(r, p, -mc_inv_n[4][0]);
@@ -316,11 +319,11 @@ MC_FN double mc_exp(double x) {
   mc_two_sum(t, -ph, &rh, &rl);
   rl = rl - mc_fma(kd, MC_LN2_128_3, pl);
   mc_fast_two_sum(rh, rl, &rh, &rl);
-  /* exp(r) - 1 = r + r^2/2 + r^3 * Q(r), Q = 1/6 + r/24 + ... + r^4/5040 */
+  /* exp(r) - 1 = r + r^2/2 + r^3 * Q(r), Q = 1/6 + r/24 + r^2/120 + r^3/720;
+     the omitted tail r^7/5040 + ... is below 2^-72 for |r| < 2^-8.5 */
   double r2h = rh * rh;
   double r2l = mc_fma(rh, rh, -r2h);
-  double q = mc_fma(rh, mc_inv_fact[7][0], mc_inv_fact[6][0]);
-  q = mc_fma(rh, q, mc_inv_fact[5][0]);
+  double q = mc_fma(rh, mc_inv_fact[6][0], mc_inv_fact[5][0]);
   q = mc_fma(rh, q, mc_inv_fact[4][0]);
   q = mc_fma(rh, q, mc_inv_fact[3][0]);
   q = (r2h * rh) * q;
