@@ -27,8 +27,11 @@
 __device__ __forceinline__ double mc_rnd_real(uint64_t &seed) {
   seed = (MC_RNG_G * seed + MC_RNG_C) & MC_MASK63;
   /* (double)seed * 2^-63, the conversion rounded to nearest and the scaling
-     exact, formed as one rounding of hi*2^-31 + lo*2^-63 (both exact) */
-  const double hi = (double)(uint32_t)(seed >> 32), lo = (double)(uint32_t)seed;
+     exact, formed as one rounding of hi*2^-31 + lo*2^-63 (both exact).  The
+     high word is taken with a bit-field extract: a plain shift and truncation
+     is widened by the compiler into a 64-bit conversion with an extra add. */
+  const double hi = (double)__builtin_amdgcn_ubfe((uint32_t)(seed >> 32), 0, 31);
+  const double lo = (double)(uint32_t)seed;
   return __builtin_fma(hi, 0x1p-31, lo * 0x1p-63);
 }
 

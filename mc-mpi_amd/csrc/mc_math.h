@@ -152,25 +152,8 @@ MC_FN_COLD double mc_log_accurate(int e, double r, int idx) {
   return rh + rl;
 }
 
-MC_FN_COLD double mc_log_special(double x) {
-  uint64_t ix = mc_asu64(x);
-  if (x != x) return x + x;                       /* NaN */
-  if ((ix << 1) == 0) return -1.0 / 0.0;           /* +-0 -> -inf */
-  if (ix >> 63) return (x - x) / (x - x);          /* x < 0 -> NaN */
-  if (ix == 0x7FF0000000000000ULL) return x;       /* +inf */
-  return 0.0; /* subnormals are handled by the caller */
-}
-
-MC_FN double mc_log(double x) {
-  uint64_t ix = mc_asu64(x);
-  int eadj = 0;
-  /* x not a positive normal finite number */
-  if (ix - 0x0010000000000000ULL >= 0x7FF0000000000000ULL - 0x0010000000000000ULL) {
-    if (ix == 0 || (ix >> 63) || ix >= 0x7FF0000000000000ULL) return mc_log_special(x);
-    /* positive subnormal: scale into the normal range */
-    ix = mc_asu64(x * 0x1p54);
-    eadj = -54;
-  }
+/* log(x) for ix = bits of a positive normal number, times 2^eadj */
+MC_FN double mc_log_normal(uint64_t ix, int eadj) {
   uint64_t tmp = ix - MC_LOG_OFF;
   int e = (int)((int64_t)tmp >> 52);
   double m = mc_asf64(ix - ((uint64_t)(int64_t)e << 52)); /* m in [sqrt(.5), sqrt(2)) */
@@ -208,6 +191,24 @@ MC_FN double mc_log(double x) {
   mc_fast_two_sum(s3, lo, &hi, &lo);
   if (mc_rounds_to_hi(hi, lo)) return hi;
   return mc_log_accurate(e, r, idx);
+}
+
+/* x not a positive normal finite number (kept out of line) */
+MC_FN_COLD double mc_log_special(double x) {
+  uint64_t ix = mc_asu64(x);
+  if (x != x) return x + x;                       /* NaN */
+  if ((ix << 1) == 0) return -1.0 / 0.0;           /* +-0 -> -inf */
+  if (ix >> 63) return (x - x) / (x - x);          /* x < 0 -> NaN */
+  if (ix == 0x7FF0000000000000ULL) return x;       /* +inf */
+  /* positive subnormal: scaled into the normal range */
+  return mc_log_normal(mc_asu64(x * 0x1p54), -54);
+}
+
+MC_FN double mc_log(double x) {
+  const uint64_t ix = mc_asu64(x);
+  if (ix - 0x0010000000000000ULL >= 0x7FF0000000000000ULL - 0x0010000000000000ULL)
+    return mc_log_special(x);
+  return mc_log_normal(ix, 0);
 }
 
 /* ------------------------------------------------------------------- exp */
@@ -313,12 +314,15 @@ MC_FN double mc_exp(double x) {
   if (__builtin_fabs(x) < 0x1p-54) return 1.0;
   double kd = __builtin_rint(x * MC_INV_LN2_128);
   int k = (int)kd;
-  double t = mc_fma(-kd, MC_LN2_128_1, x); /* exact: kd*L1 is, and so is x - kd*L1 */
-  double ph = kd * MC_LN2_128_2, pl = mc_fma(kd, MC_LN2_128_2, -ph);
-  double rh, rl;
-  mc_two_sum(t, -ph, &rh, &rl);
-  rl = rl - mc_fma(kd, MC_LN2_128_3, pl);
-  mc_fast_two_sum(rh, rl, &rh, &rl);
+  /* r = x - kd*ln2/128 as rh + rl, |kd| < 2^17, ln2/128 = A + B (+ below
+     2^-97).  t = x - kd*A is exact (kd*A has 53 bits; t is a multiple of
+     min(ulp(x), 2^-43) below 2^-8 in magnitude).  rh = RN(t - kd*B);
+     t - rh = kd*B - (rounding of rh) is below 2^-29 + 2^-62, so its
+     rounding errs by less than 2^-81; rl = RN((t - rh) - kd*B) then leaves
+     |rh + rl - r| < 2^-79. */
+  const double t = mc_fma(-kd, MC_LN2_128_A, x);
+  const double rh = mc_fma(-kd, MC_LN2_128_B, t);
+  const double rl = mc_fma(-kd, MC_LN2_128_B, t - rh);
   /* exp(r) - 1 = r + r^2/2 + r^3 * Q(r), Q = 1/6 + r/24 + r^2/120 + r^3/720;
      the omitted tail r^7/5040 + ... is below 2^-72 for |r| < 2^-8.5 */
   double r2h = rh * rh;
