@@ -40,6 +40,7 @@ struct adj_args {
 template <bool LDS, bool HAVE_SCORE>
 __global__ __launch_bounds__(1024) void k_adjoint(adj_args a, int refill_k) {
   extern __shared__ __align__(16) unsigned char smem[];
+  const mc_coef K = mc_coef_live();
   const int m = a.L.m;
   unsigned long long *tl = a.fx; /* 4m words */
   const mc_cell *cells = a.cells;
@@ -113,7 +114,7 @@ __global__ __launch_bounds__(1024) void k_adjoint(adj_args a, int refill_k) {
         const mc_cell cell = cells[c];
         bool coll;
         double di;
-        const double dw = mc_event(p, cell, a.L.dx, coll, di);
+        const double dw = mc_event(p, cell, a.L.dx, K, coll, di);
         ++ev;
         if (!HAVE_SCORE && phase == 0) {
           T = T + g[c] * dw;

@@ -79,7 +79,8 @@ __device__ __forceinline__ void fwd_tile(const bcell_args &a, const btile_k &til
                                          unsigned long long *t_lo, unsigned long long *t_hi,
                                          const mc_cell *cells, const double *g,
                                          unsigned long long *s_next, unsigned long long *s_tot,
-                                         uint32_t max_ev, int refill_k, blane &bl) {
+                                         uint32_t max_ev, int refill_k, const mc_coef &K,
+                                         blane &bl) {
   const int lane = __lane_id();
   mc_state p;
   p.index = 0;
@@ -121,7 +122,7 @@ __device__ __forceinline__ void fwd_tile(const bcell_args &a, const btile_k &til
         const int c = p.index - L.index_start;
         bool coll;
         double di;
-        const double dw = mc_event(p, cells[c], L.dx, coll, di);
+        const double dw = mc_event(p, cells[c], L.dx, K, coll, di);
         ++ev;
         col += coll ? 1u : 0u;
         mc_tally(t_lo, t_hi, a.flags + sl.cell_off, &a.ctr->status, c, dw);
@@ -154,6 +155,7 @@ __device__ __forceinline__ void fwd_tile(const bcell_args &a, const btile_k &til
 template <bool SCORE>
 __global__ __launch_bounds__(1024) void k_forward_cells_batch(bcell_args a, int refill_k) {
   extern __shared__ __align__(16) unsigned char smem[];
+  const mc_coef K = mc_coef_live();
   __shared__ int s_tile;
   __shared__ unsigned long long s_next;
   __shared__ unsigned long long s_tot[5];
@@ -193,11 +195,11 @@ __global__ __launch_bounds__(1024) void k_forward_cells_batch(bcell_args a, int 
 
     if (in_lds)
       fwd_tile<SCORE>(a, tile, sl, L, t_lds, t_lds + m, lc, lg, &s_next, s_tot, max_ev,
-                      refill_k, bl);
+                      refill_k, K, bl);
     else
       fwd_tile<SCORE>(a, tile, sl, L, a.fx + sl.cell_off, a.fx + a.total + sl.cell_off,
                       a.cells + sl.cell_off, a.g ? a.g + sl.cell_off : nullptr, &s_next, s_tot,
-                      max_ev, refill_k, bl);
+                      max_ev, refill_k, K, bl);
 
     __syncthreads();
     if (in_lds) {
@@ -238,7 +240,7 @@ __device__ __forceinline__ void adj_tile(const bcell_args &a, const btile_k &til
                                          unsigned long long *gi_lo, unsigned long long *gi_hi,
                                          const mc_cell *cells, const double *g,
                                          unsigned long long *s_next, uint32_t max_ev,
-                                         int refill_k, blane &bl) {
+                                         int refill_k, const mc_coef &K, blane &bl) {
   const int lane = __lane_id();
   uint32_t *fa = a.flags + sl.cell_off, *fi = a.flags + a.total + sl.cell_off;
   mc_state p;
@@ -289,7 +291,7 @@ __device__ __forceinline__ void adj_tile(const bcell_args &a, const btile_k &til
         const mc_cell cell = cells[c];
         bool coll;
         double di;
-        const double dw = mc_event(p, cell, L.dx, coll, di);
+        const double dw = mc_event(p, cell, L.dx, K, coll, di);
         ++ev;
         if (!HAVE_SCORE && phase == 0) {
           T = T + g[c] * dw;
@@ -327,6 +329,7 @@ __device__ __forceinline__ void adj_tile(const bcell_args &a, const btile_k &til
 template <bool HAVE_SCORE>
 __global__ __launch_bounds__(1024) void k_adjoint_cells_batch(bcell_args a, int refill_k) {
   extern __shared__ __align__(16) unsigned char smem[];
+  const mc_coef K = mc_coef_live();
   __shared__ int s_tile;
   __shared__ unsigned long long s_next;
   const int lane = __lane_id();
@@ -367,12 +370,12 @@ __global__ __launch_bounds__(1024) void k_adjoint_cells_batch(bcell_args a, int 
 
     if (in_lds)
       adj_tile<HAVE_SCORE>(a, tile, sl, L, w_lds, w_lds + m, w_lds + 2 * m, w_lds + 3 * m, lc, lg,
-                           &s_next, max_ev, refill_k, bl);
+                           &s_next, max_ev, refill_k, K, bl);
     else
       adj_tile<HAVE_SCORE>(a, tile, sl, L, a.fx + sl.cell_off, a.fx + T4 + sl.cell_off,
                            a.fx + 2 * T4 + sl.cell_off, a.fx + 3 * T4 + sl.cell_off,
                            a.cells + sl.cell_off, a.g + sl.cell_off, &s_next, max_ev, refill_k,
-                           bl);
+                           K, bl);
 
     __syncthreads();
     if (in_lds) {

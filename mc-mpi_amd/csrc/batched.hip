@@ -72,7 +72,7 @@ __device__ __forceinline__ void batch_tile(const batch_args &a, const tile_k &ti
                                            unsigned long long *t_lo, unsigned long long *t_hi,
                                            const mc_cell *cells, unsigned long long *s_next,
                                            unsigned long long *s_tot, uint32_t max_ev,
-                                           int refill_k, batch_lane &bl) {
+                                           int refill_k, const mc_coef &K, batch_lane &bl) {
   const int lane = __lane_id();
   mc_state p;
   p.index = 0;
@@ -112,7 +112,7 @@ __device__ __forceinline__ void batch_tile(const batch_args &a, const tile_k &ti
         const int c = p.index - L.index_start;
         bool coll;
         double di;
-        const double dw = mc_event(p, cells[c], L.dx, coll, di);
+        const double dw = mc_event(p, cells[c], L.dx, K, coll, di);
         ++ev;
         col += coll ? 1u : 0u;
         mc_tally(t_lo, t_hi, a.flags + sl.cell_off, &a.ctr->status, c, dw);
@@ -146,6 +146,7 @@ __device__ __forceinline__ void batch_tile(const batch_args &a, const tile_k &ti
 
 __global__ __launch_bounds__(1024) void k_batched(batch_args a, int refill_k) {
   extern __shared__ __align__(16) unsigned char smem[];
+  const mc_coef K = mc_coef_live();
   __shared__ int s_tile;
   __shared__ unsigned long long s_next;
   __shared__ unsigned long long s_tot[5];
@@ -187,14 +188,14 @@ __global__ __launch_bounds__(1024) void k_batched(batch_args a, int refill_k) {
 
     if (lds_cells)
       batch_tile<true, true>(a, tile, sl, L, t_lds, t_lds + m, lc, &s_next, s_tot, max_ev,
-                             refill_k, bl);
+                             refill_k, K, bl);
     else if (lds_tally)
       batch_tile<true, false>(a, tile, sl, L, t_lds, t_lds + m, a.cells + sl.cell_off, &s_next,
-                              s_tot, max_ev, refill_k, bl);
+                              s_tot, max_ev, refill_k, K, bl);
     else
       batch_tile<false, false>(a, tile, sl, L, a.fx + sl.cell_off,
                                a.fx + a.total_cells + sl.cell_off, a.cells + sl.cell_off,
-                               &s_next, s_tot, max_ev, refill_k, bl);
+                               &s_next, s_tot, max_ev, refill_k, K, bl);
 
     __syncthreads();
     if (lds_tally) {

@@ -113,6 +113,7 @@ template <bool LDS_TALLY, bool LDS_CELLS, bool COLL>
 __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned char *pools, int P,
                                                 int compact_every) {
   extern __shared__ __align__(16) unsigned char smem[];
+  const mc_coef K = mc_coef_live();
   /* block totals: events, collisions, left, right, killed */
   __shared__ unsigned long long s_tot[5];
   const int m = a.L.m;
@@ -207,7 +208,7 @@ __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned ch
           const int c = p.index - a.L.index_start;
           bool coll;
           double di;
-          const double dw = mc_event(p, cells[c], a.L.dx, coll, di);
+          const double dw = mc_event(p, cells[c], a.L.dx, K, coll, di);
           ++ev;
           ++pass_ev;
           col += coll ? 1u : 0u;

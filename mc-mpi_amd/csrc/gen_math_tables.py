@@ -7,13 +7,17 @@ Every value is computed here from its mathematical definition with Python's
 (hi + lo) pieces with correctly rounded conversions, so the tables carry
 about 106 bits.  Nothing is taken from any existing libm.
 
-log table (Tang-style reduction).  For x = 2^e * m with m in
-[sqrt(1/2), sqrt(2)), the index i = rint((m - 1) * 512) in [-150, 212]
-selects c_i, a 9-significant-bit approximation of 1/(1 + i/512), so that
-r = m*c_i - 1 is exact in one fma and |r| < 2^-8.  The table stores c_i and
--log(c_i) as head + mid + lo, the head on the 2^-42 grid so that e*ln2_hi +
-head is exact.  Both properties (|r| < 2^-8 and exactness of r)
-are verified below with exact rational arithmetic over every interval.
+log table (Tang-style reduction).  With tmp = bits(x) - OFF, x = 2^e * m
+where e = tmp >> 52 (arithmetic) and m in [0.7075, 1.4150); the index
+j = bits 43..51 of tmp (0..511) cuts that range into 512 buckets of equal
+width in the bit pattern (2^-10 in m below 1, 2^-9 above).  OFF is chosen
+so that m = 1 sits at the centre of its bucket, whose c is 1 (r = m - 1,
+|r| < 2^-10, so log x near 1 keeps its relative accuracy).  Every other
+bucket has c_j, a 9-significant-bit approximation of 1/m over the bucket,
+so that r = m*c_j - 1 is exact in one fma.  The table stores c_j and
+-log(c_j) as head + mid + lo, the head on the 2^-42 grid so that e*ln2_hi +
+head is exact.  |r|, the exactness of r and the FastTwoSum ordering of head
+and r are verified below with exact rational arithmetic over every bucket.
 
 exp table: 2^(j/128), j = 0..127, as double-double.
 
@@ -27,8 +31,13 @@ import struct
 
 decimal.getcontext().prec = 80
 
-LOG_SCALE = 512
-OFF = 0x3FE6A09E667F3BCD  # bits of the double nearest sqrt(1/2), rounded down
+LOG_BITS = 9
+# near the bits of sqrt(1/2); bits(1.0) - OFF = 299 * 2^43 + 2^42, so m = 1
+# is the centre of bucket 299
+OFF = 0x3FE6A40000000000
+ONE_BITS = 0x3FF0000000000000
+J_ONE = (ONE_BITS - OFF) >> 43
+assert ((ONE_BITS - OFF) & ((1 << 43) - 1)) == 1 << 42
 
 
 def f64_bits(x):
@@ -69,23 +78,75 @@ def ulp_of(x):
     return Fraction(2) ** (e - 52)
 
 
+def log_fast_error(a, b, c, head, mid, rmax):
+    """Bound on the relative error of mc_log's fast path (the double-double
+    hi + lo before the rounding test) over m in [a, b], e = 0 (for e != 0
+    the result exceeds 0.34 in magnitude and the bound is far smaller).
+
+    Absolute error terms, u = 2^-53, R = max |r|, |P| <= 0.34:
+      t = (r2h*r)*p: three roundings and the error of p's five fmas, with
+        the r2l*r term left out: below 1.7 u R^3 (counted as 2 u R^3);
+      fma(-0.5, r2l, t), serr3 + ..., lo1 + ...: three roundings of sums
+        led by t, below 1.1 u R^3 plus u(|mid| + u(|head| + 2R));
+      lo0 + serr2 and the table's dropped third part: below 2 u |mid|;
+      the series tail r^9/9 - ...: below R^9 / (9 (1 - R)).
+    Relative to min |log m| over the bucket; in the bucket of m = 1 (c = 1,
+    head = mid = 0) the terms scale with |r|^3 and |log m| >= |r| (1 - R/2),
+    so the bound is taken per unit |r|."""
+    u = Fraction(1, 2 ** 53)
+    R = rmax
+    if head == 0 and mid == 0:
+        num = Fraction(31, 10) * u * R * R + 6 * u * u + R ** 8 / (9 * (1 - R))
+        return num / (1 - R / 2)
+    absd = (Fraction(31, 10) * u * R ** 3 + 3 * u * abs(Fraction(mid))
+            + u * u * (abs(Fraction(head)) + 2 * R) + R ** 9 / (9 * (1 - R)))
+    lmin = min(abs(Decimal(a.numerator) / Decimal(a.denominator)).ln().copy_abs(),
+               abs(Decimal(b.numerator) / Decimal(b.denominator)).ln().copy_abs())
+    return absd / Fraction(lmin)
+
+
 def log_table():
-    m_lo = Fraction(bits_f64(OFF))
-    m_hi = Fraction(bits_f64(OFF + (1 << 52)))  # exclusive upper end
-    i_min = round(float((m_lo - 1) * LOG_SCALE))
-    i_max = round(float((m_hi - 1) * LOG_SCALE))
     rows = []
-    for i in range(i_min, i_max + 1):
-        mi = 1 + Fraction(i, LOG_SCALE)
-        c = Fraction(1) if i == 0 else to_nbits(1 / mi, 9)
-        # interval of m that maps to index i (ties included on both sides)
-        a = max(m_lo, mi - Fraction(1, 2 * LOG_SCALE))
-        b = min(m_hi, mi + Fraction(1, 2 * LOG_SCALE))
-        rmax = max(abs(a * c - 1), abs(b * c - 1))
-        assert rmax < Fraction(1, 256), (i, float(rmax))
-        if i != 0:
+    rmax_all = Fraction(0)
+    err_all = Fraction(0)
+    for j in range(1 << LOG_BITS):
+        # closed interval of m whose bit pattern lies in bucket j (e = 0)
+        a = Fraction(bits_f64(OFF + (j << 43)))
+        b = Fraction(bits_f64(OFF + ((j + 1) << 43) - 1))
+        if j == J_ONE:
+            assert a < 1 < b
+            cands = [Fraction(1)]
+        else:
+            assert (a >= 1) == (b >= 1), j  # no other bucket straddles m = 1
+            c0 = to_nbits(2 / (a + b), 9)
+            ce = math.floor(math.log2(c0))
+            while Fraction(2) ** ce > c0:
+                ce -= 1
+            ulp_c = Fraction(2) ** (ce - 8)
+            cands = [c0, c0 - ulp_c, c0 + ulp_c, Fraction(1)]
+        best = None
+        for c in cands:
+            rmax = max(abs(a * c - 1), abs(b * c - 1))
+            if rmax >= Fraction(1, 362):  # 1/362 < 2^-8.49: series tail bound
+                continue
+            cd = float(c)
+            assert Fraction(cd) == c
+            L = -(Decimal(cd).ln())
+            # head on the 2^-42 grid (so e*LN2_HI + head is exact for
+            # |e| < 2^11, LN2_HI having 42 bits), then two double tails
+            head = Fraction(round(Fraction(L) * 2 ** 42), 2 ** 42)
+            hi = float(head)
+            assert Fraction(hi) == head
+            rest = L - Decimal(hi)
+            mid = float(rest)
+            lo = float(rest - Decimal(mid))
+            # with e = 0 the sum hi + r is formed by FastTwoSum: it needs
+            # the exponent of hi to be at least that of r, or hi = 0
+            if hi != 0 and math.floor(math.log2(abs(hi))) < math.floor(math.log2(float(rmax))):
+                continue
             # exactness of r = m*c - 1: r is an integer multiple of
-            # ulp(m) * ulp_9bit(c); it is exact if that integer is < 2^53.
+            # ulp(m) * ulp_9bit(c); it is exact if that integer is < 2^53
+            # (c = 1 near m = 1 included: r = m - 1)
             ce = math.floor(math.log2(c))
             while Fraction(2) ** ce > c:
                 ce -= 1
@@ -93,26 +154,19 @@ def log_table():
             assert (c / ulp_c).denominator == 1
             for m_end in (a, b):
                 ulp_m = Fraction(2) ** -52 if m_end >= 1 else Fraction(2) ** -53
-                assert rmax / (ulp_m * ulp_c) < 2 ** 53, (i, "inexact r")
-            # interval must not straddle m = 1 (ulp(m) changes there)
-            assert (a >= 1) == (b > 1) or b <= 1, i
-        cd = float(c)
-        assert Fraction(cd) == c
-        L = -(Decimal(cd).ln())
-        # head on the 2^-42 grid (so e*LN2_HI + head is exact for |e| < 2^11,
-        # LN2_HI having 42 bits), then two double tails
-        head = Fraction(round(Fraction(L) * 2 ** 42), 2 ** 42)
-        hi = float(head)
-        assert Fraction(hi) == head
-        rest = L - Decimal(hi)
-        mid = float(rest)
-        lo = float(rest - Decimal(mid))
-        if i != 0:
-            # with e = 0 the sum hi + r is formed by FastTwoSum: it needs the
-            # exponent of hi to be at least that of r
-            assert math.floor(math.log2(abs(hi))) >= math.floor(math.log2(float(rmax))), i
-        rows.append((i, cd, hi, mid, lo))
-    return i_min, rows
+                if rmax / (ulp_m * ulp_c) >= 2 ** 53:
+                    break
+            else:
+                err = log_fast_error(a, b, c, hi, mid, rmax)
+                if best is None or err < best[0]:
+                    best = (err, rmax, cd, hi, mid, lo)
+        assert best is not None, j
+        err_all = max(err_all, best[0])
+        rmax_all = max(rmax_all, best[1])
+        rows.append((j,) + best[2:])
+    # mc_rounds_to_hi (mc_math.h) is stated for errors below 2^-68
+    assert err_all < Fraction(1, 2 ** 68), math.log2(err_all)
+    return float(rmax_all), float(err_all), rows
 
 
 def exp_table():
@@ -146,13 +200,13 @@ def main():
     w("#ifndef MC_MATH_TABLES_H")
     w("#define MC_MATH_TABLES_H")
     w("")
-    i_min, rows = log_table()
-    w(f"#define MC_LOG_SCALE {LOG_SCALE}.0")
-    w(f"#define MC_LOG_IMIN ({i_min})")
+    rmax, err, rows = log_table()
     w(f"#define MC_LOG_N {len(rows)}")
     w(f"#define MC_LOG_OFF 0x{OFF:016X}ULL")
-    w("/* {c_i, head, mid, lo} with -log(c_i) = head + mid + lo (head on the 2^-42")
-    w("   grid), for i = MC_LOG_IMIN .. */")
+    w(f"/* largest |r| over the table: 2^{math.log2(rmax):.3f}; bound on the relative")
+    w(f"   error of the fast path before its rounding test: 2^{math.log2(err):.2f} */")
+    w("/* {c_j, head, mid, lo} with -log(c_j) = head + mid + lo (head on the 2^-42")
+    w("   grid), for bucket j = bits 43..51 of bits(x) - MC_LOG_OFF */")
     w("MC_TAB double mc_log_tab[MC_LOG_N][4] = {")
     for (i, c, hi, mid, lo) in rows:
         w(f"  {{{hexf(c)}, {hexf(hi)}, {hexf(mid)}, {hexf(lo)}}}, /* {i} */")

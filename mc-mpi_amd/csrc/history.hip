@@ -25,6 +25,7 @@ namespace {
 template <bool LDS_TALLY, bool LDS_CELLS, bool SCORE>
 __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refill_k) {
   extern __shared__ __align__(16) unsigned char smem[];
+  const mc_coef K = mc_coef_live();
   const int m = a.L.m;
   unsigned long long *t_lo, *t_hi;
   size_t off = 0;
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refil
         const mc_cell cell = cells[c];
         bool coll;
         double di;
-        const double dw = mc_event(p, cell, a.L.dx, coll, di);
+        const double dw = mc_event(p, cell, a.L.dx, K, coll, di);
         ++ev;
         col += coll ? 1u : 0u;
         mc_tally(t_lo, t_hi, a.flags, &a.ctr->status, c, dw);

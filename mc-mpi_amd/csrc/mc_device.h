@@ -94,13 +94,22 @@ struct mc_layer_k {
 #define MC_EPS 1e-4
 #define MC_MAXREAL 1.7976931348623157e308
 
+/* the log/exp coefficients as values the compiler cannot fold; made once at
+   kernel entry and passed to every mc_event of the kernel */
+__device__ __forceinline__ mc_coef mc_coef_live() {
+  mc_coef K = mc_coef_default();
+  asm volatile("" : "+v"(K.l8), "+v"(K.l7), "+v"(K.l6), "+v"(K.l5), "+v"(K.l4), "+v"(K.l3));
+  asm volatile("" : "+v"(K.e6), "+v"(K.e5), "+v"(K.e4), "+v"(K.e3), "+v"(K.ziv));
+  return K;
+}
+
 __device__ __forceinline__ double mc_event(mc_state &p, const mc_cell cell, const double dx,
-                                           bool &collided, double &di_out) {
+                                           const mc_coef &K, bool &collided, double &di_out) {
   const double h = mc_rnd_real(p.seed);
   /* -log(h) does not depend on the cell: computed unconditionally (it is a
      pure function of h, drawn in every event), so the cell's load overlaps
      with it instead of gating it */
-  const double nlog = -mc_log(h);
+  const double nlog = -mc_log_k(h, &K);
   double di = cell.sig_i > MC_EPS ? nlog / cell.sig_i : MC_MAXREAL;
 
   int index_new;
@@ -126,7 +135,7 @@ __device__ __forceinline__ double mc_event(mc_state &p, const mc_cell cell, cons
     p.x = x_new_edge;
   }
 
-  const double dw = (1 - mc_exp(-cell.sig_a * di)) * p.wmc;
+  const double dw = (1 - mc_exp_k(-cell.sig_a * di, &K)) * p.wmc;
   p.wmc -= dw;
   p.index = index_new;
   di_out = di;

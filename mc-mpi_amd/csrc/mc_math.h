@@ -16,14 +16,17 @@
  *   - Taylor polynomials with exact rational coefficients 1/n and 1/n!;
  *   - double-double arithmetic (Dekker 1971; Knuth's TwoSum) with explicit
  *     fma, so no step depends on compiler contraction;
- *   - Ziv's strategy: a fast evaluation with relative error below 2^-70
- *     (arithmetic below 2^-72, series truncation below 2^-72; an fma in place
- *     of a multiply and an add only removes a rounding), a one-fma rounding
- *     test valid for errors below 2^-68 (mc_rounds_to_hi), and, when the test
- *     cannot decide, an
- *     all-double-double evaluation (relative error about 2^-100).  The result
- *     can differ from the correctly rounded one only when the exact value lies
- *     within about 2^-100 (relative) of a rounding boundary.
+ *   - Ziv's strategy: a fast evaluation with a bounded relative error, a
+ *     one-fma rounding test valid for errors below 2^-68 (mc_rounds_to_hi),
+ *     and, when the test cannot decide, an all-double-double evaluation
+ *     (relative error about 2^-100).  The fast log is below 2^-68.6: a bound
+ *     gen_math_tables.py computes for every table row and asserts; the
+ *     largest rows are next to x = 1, where log x is small.  The fast exp is
+ *     below 2^-71.8 (series truncation below 2^-72, argument reduction below
+ *     2^-79, roundings below 2^-76).  An fma in place of a multiply and an
+ *     add only removes a rounding.  The result can differ from the correctly
+ *     rounded one only when the exact value lies within about 2^-100
+ *     (relative) of a rounding boundary.
  *
  * The header compiles as HIP device code (hipcc) or as plain host C/C++ (gcc)
  * for CPU-side tests; the arithmetic is the same IEEE-754 binary64 sequence in
@@ -98,7 +101,7 @@ MC_FN int mc_round_ok(double hi, double lo, double err) {
 
 /* The same certificate in one fma, for hi + lo (hi = RN(hi + lo)) within
    relative error eps of the exact value Y, eps < 2^-68 (the fast paths are
-   below 2^-72).  Let b be the distance from hi to its rounding boundary on
+   below 2^-68.6 for log and 2^-71.8 for exp).  Let b be the distance from hi to its rounding boundary on
    the side of lo: b >= 2^-54 |hi| (half an ulp of hi, or a quarter below a
    power of two).  RN(hi + e lo) == hi with e = 1 + 2^-13 (one rounding, at
    that boundary, ties included) gives |lo| <= b/e, so on lo's side
@@ -107,6 +110,30 @@ MC_FN int mc_round_ok(double hi, double lo, double err) {
    strictly inside hi's rounding interval and rounds to hi. */
 #define MC_ZIV_E (1.0 + 0x1p-13)
 MC_FN int mc_rounds_to_hi(double hi, double lo) { return mc_fma(lo, MC_ZIV_E, hi) == hi; }
+
+/* The fast paths' polynomial coefficients and the certificate's factor e.
+   mc_log and mc_exp use them as constants.  Kernels pass a copy whose values
+   the compiler cannot fold (made once at kernel entry, mc_coef_live in
+   mc_device.h) to mc_log_k / mc_exp_k: each value then stays in one 64-bit
+   register across the particle loop and each Horner step compiles to one
+   fma, where folded constants are rebuilt from 32-bit halves on every use
+   and copied into the accumulator first.  Values and results are the same
+   either way. */
+typedef struct {
+  double l8, l7, l6, l5, l4, l3; /* -1/8, 1/7, -1/6, 1/5, -1/4, 1/3 */
+  double e6, e5, e4, e3;         /* 1/6!, 1/5!, 1/4!, 1/3! */
+  double ziv;                    /* MC_ZIV_E */
+} mc_coef;
+
+MC_FN mc_coef mc_coef_default(void) {
+  mc_coef K;
+  K.l8 = -mc_inv_n[8][0]; K.l7 = mc_inv_n[7][0]; K.l6 = -mc_inv_n[6][0];
+  K.l5 = mc_inv_n[5][0];  K.l4 = -mc_inv_n[4][0]; K.l3 = mc_inv_n[3][0];
+  K.e6 = mc_inv_fact[6][0]; K.e5 = mc_inv_fact[5][0];
+  K.e4 = mc_inv_fact[4][0]; K.e3 = mc_inv_fact[3][0];
+  K.ziv = MC_ZIV_E;
+  return K;
+}
 
 /* ------------------------------------------------------------------- log */
 
@@ -153,25 +180,39 @@ MC_FN_COLD double mc_log_accurate(int e, double r, int idx) {
 }
 
 /* log(x) for ix = bits of a positive normal number, times 2^eadj */
-MC_FN double mc_log_normal(uint64_t ix, int eadj) {
-  uint64_t tmp = ix - MC_LOG_OFF;
+MC_FN double mc_log_normal(uint64_t ix, int eadj, const mc_coef *K) {
+  const uint64_t tmp = ix - MC_LOG_OFF;
+  /* e = tmp >> 52 (arithmetic), m = x / 2^e in [sqrt(.5), sqrt(2)) */
+  /* table row: bucket j = bits 43..51 of tmp (512 buckets of m) */
+#if defined(__HIP_DEVICE_COMPILE__)
+  /* the same on the high words, in 32-bit operations (a 64-bit shift is
+     otherwise turned into a 64-bit integer-to-double conversion) */
+  const uint32_t thi = (uint32_t)(tmp >> 32);
+  int e = __builtin_amdgcn_sbfe((int)thi, 20, 12);
+  const uint32_t mhi = (uint32_t)(ix >> 32) - (thi & 0xFFF00000u);
+  const double m = mc_asf64(((uint64_t)mhi << 32) | (uint32_t)ix);
+  const unsigned idx = __builtin_amdgcn_ubfe(thi, 11, 9);
+#else
   int e = (int)((int64_t)tmp >> 52);
-  double m = mc_asf64(ix - ((uint64_t)(int64_t)e << 52)); /* m in [sqrt(.5), sqrt(2)) */
+  const double m = mc_asf64(ix - ((uint64_t)(int64_t)e << 52));
+  const unsigned idx = (unsigned)(tmp >> 43) & 511u;
+#endif
   e += eadj;
-  int idx = (int)__builtin_rint((m - 1.0) * MC_LOG_SCALE) - MC_LOG_IMIN;
   double c = mc_log_tab[idx][0];
-  double r = mc_fma(m, c, -1.0); /* exact, |r| < 2^-8 */
+  double r = mc_fma(m, c, -1.0); /* exact, |r| < 2^-8.67 */
 
   /* log1p(r) = r - r^2/2 + r^3 * P(r), P = 1/3 - r/4 + ... - r^5/8; the
-     omitted tail (r^9/9 - ...) is below 2^-74 of the result for |r| < 2^-8.4
-     (the largest |r| of the table; near x = 1, |r| < 2^-10 and log x ~ r) */
+     omitted tail (r^9/9 - ...) is below R^9/(9(1 - R)) for the table's
+     largest |r|, R = 2^-8.67.  Near x = 1 the rows have c = 1 (r = m - 1,
+     log x ~ r); gen_math_tables.py bounds the relative error of this whole
+     fast path per row (all terms, roundings included). */
   double r2h = r * r;
   double r2l = mc_fma(r, r, -r2h);
-  double p = mc_fma(r, -mc_inv_n[8][0], mc_inv_n[7][0]);
-  p = mc_fma(r, p, -mc_inv_n[6][0]);
-  p = mc_fma(r, p, mc_inv_n[5][0]);
-  p = mc_fma(r, p, -mc_inv_n[4][0]);
-  p = mc_fma(r, p, mc_inv_n[3][0]);
+  double p = mc_fma(r, K->l8, K->l7);
+  p = mc_fma(r, p, K->l6);
+  p = mc_fma(r, p, K->l5);
+  p = mc_fma(r, p, K->l4);
+  p = mc_fma(r, p, K->l3);
   double t = (r2h * r) * p;
 
   /* e*ln2 + (-log c) + r - r^2/2 + t, accumulated in double-double.
@@ -189,8 +230,8 @@ MC_FN double mc_log_normal(uint64_t ix, int eadj) {
   lo = lo + (serr3 + mc_fma(-0.5, r2l, t));
   double hi;
   mc_fast_two_sum(s3, lo, &hi, &lo);
-  if (mc_rounds_to_hi(hi, lo)) return hi;
-  return mc_log_accurate(e, r, idx);
+  if (mc_fma(lo, K->ziv, hi) == hi) return hi; /* mc_rounds_to_hi */
+  return mc_log_accurate(e, r, (int)idx);
 }
 
 /* x not a positive normal finite number (kept out of line) */
@@ -201,14 +242,20 @@ MC_FN_COLD double mc_log_special(double x) {
   if (ix >> 63) return (x - x) / (x - x);          /* x < 0 -> NaN */
   if (ix == 0x7FF0000000000000ULL) return x;       /* +inf */
   /* positive subnormal: scaled into the normal range */
-  return mc_log_normal(mc_asu64(x * 0x1p54), -54);
+  const mc_coef K = mc_coef_default();
+  return mc_log_normal(mc_asu64(x * 0x1p54), -54, &K);
 }
 
-MC_FN double mc_log(double x) {
+MC_FN double mc_log_k(double x, const mc_coef *K) {
   const uint64_t ix = mc_asu64(x);
   if (ix - 0x0010000000000000ULL >= 0x7FF0000000000000ULL - 0x0010000000000000ULL)
     return mc_log_special(x);
-  return mc_log_normal(ix, 0);
+  return mc_log_normal(ix, 0, K);
+}
+
+MC_FN double mc_log(double x) {
+  const mc_coef K = mc_coef_default();
+  return mc_log_k(x, &K);
 }
 
 /* ------------------------------------------------------------------- exp */
@@ -307,7 +354,7 @@ MC_FN_COLD double mc_exp_special(double x) {
   return n * 0x1p-1074;
 }
 
-MC_FN double mc_exp(double x) {
+MC_FN double mc_exp_k(double x, const mc_coef *K) {
   if (!(x > -708.3 && x < 709.7)) return mc_exp_special(x);
   /* |x| < 2^-54: exp(x) rounds to 1 (1 + x lies strictly inside the
      rounding interval of 1, and so does exp(x) = 1 + x + O(x^2)). */
@@ -327,9 +374,9 @@ MC_FN double mc_exp(double x) {
      the omitted tail r^7/5040 + ... is below 2^-72 for |r| < 2^-8.5 */
   double r2h = rh * rh;
   double r2l = mc_fma(rh, rh, -r2h);
-  double q = mc_fma(rh, mc_inv_fact[6][0], mc_inv_fact[5][0]);
-  q = mc_fma(rh, q, mc_inv_fact[4][0]);
-  q = mc_fma(rh, q, mc_inv_fact[3][0]);
+  double q = mc_fma(rh, K->e6, K->e5);
+  q = mc_fma(rh, q, K->e4);
+  q = mc_fma(rh, q, K->e3);
   q = (r2h * rh) * q;
   double ah, al;
   mc_fast_two_sum(rh, 0.5 * r2h, &ah, &al);
@@ -345,11 +392,16 @@ MC_FN double mc_exp(double x) {
   yl = yl + mc_fma(th, el, tl * eh);
   mc_fast_two_sum(yh, yl, &yh, &yl);
   double sc = mc_pow2((k - j) / 128);
-  if (mc_rounds_to_hi(yh, yl)) return yh * sc;
+  if (mc_fma(yl, K->ziv, yh) == yh) return yh * sc; /* mc_rounds_to_hi */
   if (__builtin_fabs(x) < 0x1p-30) return mc_exp_tiny(x);
   int kk;
   mc_exp_dd(x, 0.0, &yh, &yl, &kk);
   return (yh + yl) * mc_pow2(kk);
+}
+
+MC_FN double mc_exp(double x) {
+  const mc_coef K = mc_coef_default();
+  return mc_exp_k(x, &K);
 }
 
 #endif /* MC_MATH_H */
