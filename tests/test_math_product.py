@@ -90,3 +90,35 @@ def test_exp_structured_tiny_and_subnormal(pm, oracle):
 def test_exp_full_range(pm, oracle):
     x = (np.random.default_rng(6).random(300_000) - 0.5) * 1500.0
     assert np.all(_same(pm("exp", x), oracle.math_eval("exp", x)))
+
+
+def test_exp_near_multiples_of_ln2_over_128(pm, oracle):
+    """Arguments next to k*ln2/128, where the reduced argument r nearly
+    cancels and the two-fma reduction's error bound is tightest."""
+    rng = np.random.default_rng(7)
+    k = rng.integers(-130_000, 131_000, 300_000).astype(np.float64)
+    off = (rng.integers(-1024, 1024, 300_000).astype(np.float64)
+           * 2.0 ** -rng.integers(40, 70, 300_000).astype(np.float64))
+    x = k * (np.log(2.0) / 128.0) + off
+    x = x[(x > -708.3) & (x < 709.7)]
+    assert np.all(_same(pm("exp", x), oracle.math_eval("exp", x)))
+
+
+def test_log_rows_next_to_one(pm, oracle):
+    """The rows with the largest fast-path error bound are the buckets next to
+    x = 1 (|log x| small); sample them densely, with every exponent."""
+    rng = np.random.default_rng(8)
+    m = 1.0 + (rng.random(300_000) - 0.5) * 2.0 ** -5
+    x = np.ldexp(m, rng.integers(-3, 4, 300_000))
+    assert np.all(_same(pm("log", x), oracle.math_eval("log", x)))
+
+
+def test_math_tables_regenerate_identically():
+    """gen_math_tables.py recomputes every table entry from its definition and
+    asserts, per log-table row, the exactness of r = m*c - 1, the FastTwoSum
+    ordering and the fast path's error bound (below 2^-68); its output must
+    be the committed header byte for byte."""
+    gen = os.path.join(ROOT, "mc-mpi_amd", "csrc", "gen_math_tables.py")
+    out = subprocess.run(["python3", gen], check=True, capture_output=True, text=True).stdout
+    with open(os.path.join(ROOT, "mc-mpi_amd", "csrc", "mc_math_tables.h")) as f:
+        assert out == f.read()
