@@ -143,8 +143,11 @@ __device__ __forceinline__ double mc_event(mc_state &p, const mc_cell cell, cons
 }
 
 /* src/layer.cpp:195-197: the history continues while this holds */
+/* index_start <= index < index_start + m as one unsigned compare (the
+   difference wraps to a large value below index_start) */
 __device__ __forceinline__ bool mc_alive(const mc_state &p, const mc_layer_k &L) {
-  return (p.wmc >= L.min_weight) && (p.index < L.index_start + L.m) && (p.index >= L.index_start);
+  return (p.wmc >= L.min_weight) &&
+         ((unsigned)p.index - (unsigned)L.index_start < (unsigned)L.m);
 }
 
 /* src/layer.cpp:202-217: exit code of a finished history */

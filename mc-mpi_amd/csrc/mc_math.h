@@ -248,7 +248,9 @@ MC_FN_COLD double mc_log_special(double x) {
 
 MC_FN double mc_log_k(double x, const mc_coef *K) {
   const uint64_t ix = mc_asu64(x);
-  if (ix - 0x0010000000000000ULL >= 0x7FF0000000000000ULL - 0x0010000000000000ULL)
+  /* x not a positive normal finite number; the subtrahend's low word is
+     zero, so the test needs only the high word */
+  if ((uint32_t)(ix >> 32) - 0x00100000u >= 0x7FF00000u - 0x00100000u)
     return mc_log_special(x);
   return mc_log_normal(ix, 0, K);
 }
