@@ -44,9 +44,22 @@ BYTES_PER_EVENT = 72  # algorithmic: the 36-byte particle state read + written p
 
 # HBM bytes per event measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
 # this workload (1e7 particles), scaled by the copy-kernel calibration (read
-# x2.0, write x1.0); profiles/history_config2_counters.md and
-# profiles/event_config2_hbm_counters.md hold the counter runs.
-MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0602, "event": 88.2}
+# x2.0, write x1.0); the counter runs are summarised under profiles/.
+MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0613, "event": 88.3}
+# VALU wave-instructions per event (rocprofv3 --pmc SQ_INSTS_VALU, same runs).
+# gfx950 issues one wave64 VALU instruction per SIMD every 4 cycles:
+# 256 CUs x 4 SIMDs x 2.4 GHz / 4 = 6.144e11 wave-instructions/s.
+MEASURED_VALU_PER_EVENT = {"history": 2.879, "event": 4.542}
+VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
+
+
+def valu_rate(kernel, events, seconds):
+    w = MEASURED_VALU_PER_EVENT.get(kernel)
+    if w is None:
+        return None
+    a = w * events / seconds
+    return {"wave_instructions_per_event": w, "achieved": a, "peak": VALU_PEAK_WAVE_INSTR,
+            "unit": "wave-instructions/s", "frac": a / VALU_PEAK_WAVE_INSTR}
 
 
 def log(msg):
@@ -223,7 +236,11 @@ def run_cycle(ctx, args, torch, mcgpu, configs):
     if r.returncode != 0:
         raise RuntimeError(f"mcgpu_run failed: {r.stderr[-500:]}")
     secs = float(r.stdout.split()[0])
-    fields = dict(kv.split(" ", 1) for kv in r.stderr.strip().splitlines()[-1].split(", "))
+    # the summary line; a profiler attached to the child adds lines of its own
+    summary = [ln for ln in r.stderr.splitlines() if ln.startswith("events ")]
+    if not summary:
+        raise RuntimeError(f"mcgpu_run printed no summary: {r.stderr[-500:]}")
+    fields = dict(kv.split(" ", 1) for kv in summary[-1].split(", "))
     events = int(fields["events"])
     out = {"workload": f"config.yaml source, 1000 cells, {args.cycle_particles} particles, "
                        "1e5 per cycle, 5 ranks on one GPU (mcgpu_run --ranks 5)",
@@ -440,8 +457,9 @@ def main():
                      "note": ("72 algorithmic B/event (36-byte state read + written, the "
                               "event formulation); the history kernel keeps the state in "
                               "registers (measured 0.06 B/event) and is bound by FP64 VALU "
-                              "issue, hence frac > 1") if args.kernel == "history" else
-                             "72 algorithmic B/event; measured 88 B/event (44-byte pool slots)"},
+                              "issue (see valu), hence frac > 1") if args.kernel == "history" else
+                             "72 algorithmic B/event; measured 88 B/event (44-byte pool slots)",
+                     "valu": valu_rate(args.kernel, ev_step, k_avg_s)},
         "cpu_baseline": None,
         "variants": {},
     }
@@ -451,7 +469,8 @@ def main():
         tr_o = MEASURED_TRAFFIC_PER_EVENT.get(other["kernel"])
         other["roofline"] = {"bound": "hbm", "achieved": ach_o, "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": ach_o / HBM_PEAK_GBS,
-                             "traffic": None if tr_o is None else tr_o * ev_o}
+                             "traffic": None if tr_o is None else tr_o * ev_o,
+                             "valu": valu_rate(other["kernel"], ev_o, t_o)}
         out["variants"][other["kernel"]] = other
     if not args.plain_only:
         runs = {"host_api": run_host_api, "cycle": run_cycle, "batched": run_batched,
