@@ -105,25 +105,25 @@ __device__ __forceinline__ mc_coef mc_coef_live() {
 
 __device__ __forceinline__ double mc_event(mc_state &p, const mc_cell cell, const double dx,
                                            const mc_coef &K, bool &collided, double &di_out) {
+  /* The draw and the log's table row come first; the edge quotient, which
+     does not depend on them, is formed while the row's loads are in flight
+     (the empty asm keeps it there instead of letting the compiler sink it
+     past the log).  Both quotients are formed unconditionally and then
+     selected; the discarded one (|mu| <= EPS, or sig_i <= EPS) may be inf
+     or NaN and is never used. */
   const double h = mc_rnd_real(p.seed);
-  /* -log(h) does not depend on the cell: computed unconditionally (it is a
-     pure function of h, drawn in every event), so the cell's load overlaps
-     with it instead of gating it */
-  const double nlog = -mc_log_k(h, &K);
-  double di = cell.sig_i > MC_EPS ? nlog / cell.sig_i : MC_MAXREAL;
+  const mc_log_row lrow = mc_log_start(h);
 
-  int index_new;
-  double x_new_edge;
-  if (p.mu < 0) {
-    index_new = p.index - 1;
-    x_new_edge = p.index * dx;
-  } else {
-    index_new = p.index + 1;
-    x_new_edge = (p.index + 1) * dx;
-  }
+  const bool back = p.mu < 0;
+  int index_new = back ? p.index - 1 : p.index + 1;
+  const double x_new_edge = (back ? p.index : p.index + 1) * dx;
+  const double q_edge = (x_new_edge - p.x) / p.mu;
+  asm volatile("" ::"v"(q_edge));
+  double di_edge = (p.mu < -MC_EPS || MC_EPS < p.mu) ? q_edge : MC_MAXREAL;
 
-  double di_edge = MC_MAXREAL;
-  if (p.mu < -MC_EPS || MC_EPS < p.mu) di_edge = (x_new_edge - p.x) / p.mu;
+  const double nlog = -mc_log_finish(h, &lrow, &K);
+  const double q_i = nlog / cell.sig_i;
+  double di = cell.sig_i > MC_EPS ? q_i : MC_MAXREAL;
 
   collided = di < di_edge;
   if (collided) {

@@ -179,27 +179,43 @@ MC_FN_COLD double mc_log_accurate(int e, double r, int idx) {
   return rh + rl;
 }
 
-/* log(x) for ix = bits of a positive normal number, times 2^eadj */
-MC_FN double mc_log_normal(uint64_t ix, int eadj, const mc_coef *K) {
+/* log in two halves.  mc_log_lookup splits x (bits ix, times 2^eadj) and
+   loads its table row; mc_log_eval finishes from the row.  A caller with
+   independent work can put it between the two, so that the row's loads are
+   in flight meanwhile. */
+typedef struct {
+  double m, c, head, mid;
+  int e;
+  unsigned idx;
+} mc_log_row;
+
+MC_FN mc_log_row mc_log_lookup(uint64_t ix, int eadj) {
+  mc_log_row R;
   const uint64_t tmp = ix - MC_LOG_OFF;
-  /* e = tmp >> 52 (arithmetic), m = x / 2^e in [sqrt(.5), sqrt(2)) */
-  /* table row: bucket j = bits 43..51 of tmp (512 buckets of m) */
+  /* e = tmp >> 52 (arithmetic), m = x / 2^e in [0.7075, 1.4150);
+     table row: bucket j = bits 43..51 of tmp (512 buckets of m) */
 #if defined(__HIP_DEVICE_COMPILE__)
   /* the same on the high words, in 32-bit operations (a 64-bit shift is
      otherwise turned into a 64-bit integer-to-double conversion) */
   const uint32_t thi = (uint32_t)(tmp >> 32);
-  int e = __builtin_amdgcn_sbfe((int)thi, 20, 12);
+  R.e = __builtin_amdgcn_sbfe((int)thi, 20, 12) + eadj;
   const uint32_t mhi = (uint32_t)(ix >> 32) - (thi & 0xFFF00000u);
-  const double m = mc_asf64(((uint64_t)mhi << 32) | (uint32_t)ix);
-  const unsigned idx = __builtin_amdgcn_ubfe(thi, 11, 9);
+  R.m = mc_asf64(((uint64_t)mhi << 32) | (uint32_t)ix);
+  R.idx = __builtin_amdgcn_ubfe(thi, 11, 9);
 #else
-  int e = (int)((int64_t)tmp >> 52);
-  const double m = mc_asf64(ix - ((uint64_t)(int64_t)e << 52));
-  const unsigned idx = (unsigned)(tmp >> 43) & 511u;
+  const int e = (int)((int64_t)tmp >> 52);
+  R.e = e + eadj;
+  R.m = mc_asf64(ix - ((uint64_t)(int64_t)e << 52));
+  R.idx = (unsigned)(tmp >> 43) & 511u;
 #endif
-  e += eadj;
-  double c = mc_log_tab[idx][0];
-  double r = mc_fma(m, c, -1.0); /* exact, |r| < 2^-8.67 */
+  R.c = mc_log_tab[R.idx][0];
+  R.head = mc_log_tab[R.idx][1];
+  R.mid = mc_log_tab[R.idx][2];
+  return R;
+}
+
+MC_FN double mc_log_eval(const mc_log_row *R, const mc_coef *K) {
+  const double r = mc_fma(R->m, R->c, -1.0); /* exact, |r| < 2^-8.67 */
 
   /* log1p(r) = r - r^2/2 + r^3 * P(r), P = 1/3 - r/4 + ... - r^5/8; the
      omitted tail (r^9/9 - ...) is below R^9/(9(1 - R)) for the table's
@@ -220,10 +236,10 @@ MC_FN double mc_log_normal(uint64_t ix, int eadj, const mc_coef *K) {
      next two sums are FastTwoSums: |s| >= |r| unless s == 0 (e != 0 gives
      |s| > 0.34; e == 0 is checked per table row by gen_math_tables.py), and
      s + r then dominates r^2/2. */
-  double ed = (double)e;
+  double ed = (double)R->e;
   double s2, serr2, s3, serr3;
-  const double s = ed * MC_LN2_HI + mc_log_tab[idx][1];
-  double lo = mc_fma(ed, MC_LN2_MID, mc_log_tab[idx][2]);
+  const double s = ed * MC_LN2_HI + R->head;
+  double lo = mc_fma(ed, MC_LN2_MID, R->mid);
   mc_fast_two_sum(s, r, &s2, &serr2);
   lo = lo + serr2;
   mc_fast_two_sum(s2, -0.5 * r2h, &s3, &serr3);
@@ -231,7 +247,7 @@ MC_FN double mc_log_normal(uint64_t ix, int eadj, const mc_coef *K) {
   double hi;
   mc_fast_two_sum(s3, lo, &hi, &lo);
   if (mc_fma(lo, K->ziv, hi) == hi) return hi; /* mc_rounds_to_hi */
-  return mc_log_accurate(e, r, (int)idx);
+  return mc_log_accurate(R->e, r, (int)R->idx);
 }
 
 /* x not a positive normal finite number (kept out of line) */
@@ -243,16 +259,30 @@ MC_FN_COLD double mc_log_special(double x) {
   if (ix == 0x7FF0000000000000ULL) return x;       /* +inf */
   /* positive subnormal: scaled into the normal range */
   const mc_coef K = mc_coef_default();
-  return mc_log_normal(mc_asu64(x * 0x1p54), -54, &K);
+  const mc_log_row R = mc_log_lookup(mc_asu64(x * 0x1p54), -54);
+  return mc_log_eval(&R, &K);
+}
+
+/* x not a positive normal finite number; the subtrahend's low word is zero,
+   so the test needs only the high word */
+MC_FN int mc_log_is_special(double x) {
+  return (uint32_t)(mc_asu64(x) >> 32) - 0x00100000u >= 0x7FF00000u - 0x00100000u;
+}
+
+/* The table row of x, or of 1.0 in place of a special x (no branch: the
+   special result replaces the value in mc_log_finish). */
+MC_FN mc_log_row mc_log_start(double x) {
+  return mc_log_lookup(mc_log_is_special(x) ? 0x3FF0000000000000ULL : mc_asu64(x), 0);
+}
+
+MC_FN double mc_log_finish(double x, const mc_log_row *R, const mc_coef *K) {
+  const double y = mc_log_eval(R, K);
+  return mc_log_is_special(x) ? mc_log_special(x) : y;
 }
 
 MC_FN double mc_log_k(double x, const mc_coef *K) {
-  const uint64_t ix = mc_asu64(x);
-  /* x not a positive normal finite number; the subtrahend's low word is
-     zero, so the test needs only the high word */
-  if ((uint32_t)(ix >> 32) - 0x00100000u >= 0x7FF00000u - 0x00100000u)
-    return mc_log_special(x);
-  return mc_log_normal(ix, 0, K);
+  const mc_log_row R = mc_log_start(x);
+  return mc_log_finish(x, &R, K);
 }
 
 MC_FN double mc_log(double x) {
