@@ -100,17 +100,29 @@ __device__ __forceinline__ mc_coef mc_coef_live() {
   mc_coef K = mc_coef_default();
   asm volatile("" : "+v"(K.l8), "+v"(K.l7), "+v"(K.l6), "+v"(K.l5), "+v"(K.l4), "+v"(K.l3));
   asm volatile("" : "+v"(K.e6), "+v"(K.e5), "+v"(K.e4), "+v"(K.e3), "+v"(K.ziv));
+  asm volatile("" : "+v"(K.a6), "+v"(K.a5), "+v"(K.a3));
   return K;
 }
 
+/* One particle_step (src/layer.cpp:129-180).  The reference draws the
+   flight length di = -log(h) / sig_i and the edge distance di_edge, and the
+   particle collides iff di < di_edge (both correctly rounded quotients
+   here).  On fine meshes most events cross the cell edge, and then di itself
+   is never used: the event first decides the comparison with a plain-double
+   log (mc_log_approx, within 2^-48) and a product, with a 2^-40 margin:
+     -log(h) > sig_i * di_edge * (1 + 2^-40)  (computed in floating point)
+   implies the exact -log(h) / sig_i > di_edge, so RN(di) >= di_edge and the
+   particle crosses.  Only when that is not certain (a collision, a near tie,
+   a special h) are the correctly rounded log and the division formed and
+   compared exactly, as the reference does.  Results are identical either
+   way. */
 __device__ __forceinline__ double mc_event(mc_state &p, const mc_cell cell, const double dx,
                                            const mc_coef &K, bool &collided, double &di_out) {
   /* The draw and the log's table row come first; the edge quotient, which
      does not depend on them, is formed while the row's loads are in flight
      (the empty asm keeps it there instead of letting the compiler sink it
-     past the log).  Both quotients are formed unconditionally and then
-     selected; the discarded one (|mu| <= EPS, or sig_i <= EPS) may be inf
-     or NaN and is never used. */
+     past the log).  It is formed unconditionally and then selected; when
+     |mu| <= EPS the discarded quotient may be inf or NaN and is never used. */
   const double h = mc_rnd_real(p.seed);
   const mc_log_row lrow = mc_log_start(h);
 
@@ -119,19 +131,26 @@ __device__ __forceinline__ double mc_event(mc_state &p, const mc_cell cell, cons
   const double x_new_edge = (back ? p.index : p.index + 1) * dx;
   const double q_edge = (x_new_edge - p.x) / p.mu;
   asm volatile("" ::"v"(q_edge));
-  double di_edge = (p.mu < -MC_EPS || MC_EPS < p.mu) ? q_edge : MC_MAXREAL;
+  const double di_edge = (p.mu < -MC_EPS || MC_EPS < p.mu) ? q_edge : MC_MAXREAL;
 
-  const double nlog = -mc_log_finish(h, &lrow, &K);
-  const double q_i = nlog / cell.sig_i;
-  double di = cell.sig_i > MC_EPS ? q_i : MC_MAXREAL;
-
-  collided = di < di_edge;
+  const double nlog_a = -mc_log_approx(&lrow, &K);
+  const double reach = cell.sig_i * di_edge;
+  /* bitwise, not short-circuit: no branch around the approximate log */
+  const bool sure_cross = !mc_log_is_special(h) &
+                          (!(cell.sig_i > MC_EPS) | (nlog_a > mc_fma(reach, 0x1p-40, reach)));
+  double di = di_edge;
+  collided = false;
+  if (!sure_cross) {
+    const double nlog = -mc_log_finish(h, &lrow, &K);
+    const double dq = cell.sig_i > MC_EPS ? nlog / cell.sig_i : MC_MAXREAL;
+    collided = dq < di_edge;
+    if (collided) di = dq;
+  }
   if (collided) {
     index_new = p.index;
     p.x += di * p.mu;
     p.mu = 2 * mc_rnd_real(p.seed) - 1;
   } else {
-    di = di_edge;
     p.x = x_new_edge;
   }
 

@@ -122,3 +122,40 @@ def test_math_tables_regenerate_identically():
     out = subprocess.run(["python3", gen], check=True, capture_output=True, text=True).stdout
     with open(os.path.join(ROOT, "mc-mpi_amd", "csrc", "mc_math_tables.h")) as f:
         assert out == f.read()
+
+
+APPROX_SHIM = r"""
+#include <stdint.h>
+#include "mc_math.h"
+void pm_log_approx(const double *x, long n, double *y) {
+  const mc_coef K = mc_coef_default();
+  for (long i = 0; i < n; ++i) { const mc_log_row R = mc_log_start(x[i]); y[i] = mc_log_approx(&R, &K); }
+}
+"""
+
+
+def test_log_approx_error_bound(tmp_path, oracle):
+    """mc_log_approx decides crossings with a 2^-40 margin; it must stay
+    within its stated 2^-48 of the correctly rounded log over every table
+    row (dense near x = 1, where |log x| is smallest) and every exponent."""
+    src = tmp_path / "pa.c"
+    src.write_text(APPROX_SHIM)
+    so = tmp_path / "libpa.so"
+    subprocess.run(["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-ffp-contract=off",
+                    "-I", os.path.join(ROOT, "mc-mpi_amd", "csrc"), "-o", str(so), str(src), "-lm"],
+                   check=True)
+    L = C.CDLL(str(so))
+    L.pm_log_approx.argtypes = [C.c_void_p, C.c_long, C.c_void_p]
+    rng = np.random.default_rng(9)
+    s = rng.integers(1, 2 ** 63, 300_000, dtype=np.uint64)
+    h = s.astype(np.float64) * 2.0 ** -63                      # rnd_real outputs
+    near = 1.0 + (rng.random(300_000) - 0.5) * 2.0 ** -4        # rows next to 1
+    tiny = 1.0 - rng.integers(1, 2 ** 20, 50_000).astype(np.float64) * 2.0 ** -53
+    x = np.concatenate([h, near, np.ldexp(near, rng.integers(-60, 60, 300_000)), tiny])
+    x = x[x != 1.0]
+    y = np.empty_like(x)
+    L.pm_log_approx(x.ctypes.data, len(x), y.ctypes.data)
+    ref = oracle.math_eval("log", x)
+    rel = np.abs(y - ref) / np.abs(ref)
+    print("max rel error 2^%.2f" % np.log2(rel.max()))
+    assert rel.max() < 2.0 ** -48, np.log2(rel.max())
