@@ -88,6 +88,7 @@ __global__ __launch_bounds__(1024) void k_adjoint(adj_args a, int refill_k) {
             p.seed = a.init.seed[i];
             p.x = a.init.x[i];
             p.mu = a.init.mu[i];
+            mc_refresh_rcp(p);
             p.wmc = a.init.wmc[i];
             p.index = a.init.index[i];
             idx = i;
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(1024) void k_adjoint(adj_args a, int refill_k) {
         const mc_cell cell = cells[c];
         bool coll;
         double di;
-        const double dw = mc_event(p, cell, a.L.dx, K, coll, di);
+        const double dw = mc_event<true>(p, cell, a.L.dx, K, coll, di);
         ++ev;
         if (!HAVE_SCORE && phase == 0) {
           T = T + g[c] * dw;
@@ -136,6 +137,7 @@ __global__ __launch_bounds__(1024) void k_adjoint(adj_args a, int refill_k) {
           p.seed = a.init.seed[idx];
           p.x = a.init.x[idx];
           p.mu = a.init.mu[idx];
+          mc_refresh_rcp(p);
           p.wmc = a.init.wmc[idx];
           p.index = a.init.index[idx];
           ev = 0;

@@ -103,6 +103,7 @@ __device__ __forceinline__ void fwd_tile(const bcell_args &a, const btile_k &til
             p.seed = a.bank.seed[i];
             p.x = a.bank.x[i];
             p.mu = a.bank.mu[i];
+            mc_refresh_rcp(p);
             p.wmc = a.bank.wmc[i];
             p.index = a.bank.index[i];
             idx = i;
@@ -122,7 +123,7 @@ __device__ __forceinline__ void fwd_tile(const bcell_args &a, const btile_k &til
         const int c = p.index - L.index_start;
         bool coll;
         double di;
-        const double dw = mc_event(p, cells[c], L.dx, K, coll, di);
+        const double dw = mc_event<true>(p, cells[c], L.dx, K, coll, di);
         ++ev;
         col += coll ? 1u : 0u;
         mc_tally(t_lo, t_hi, a.flags + sl.cell_off, &a.ctr->status, c, dw);
@@ -265,6 +266,7 @@ __device__ __forceinline__ void adj_tile(const bcell_args &a, const btile_k &til
             p.seed = a.bank.seed[i];
             p.x = a.bank.x[i];
             p.mu = a.bank.mu[i];
+            mc_refresh_rcp(p);
             p.wmc = a.bank.wmc[i];
             p.index = a.bank.index[i];
             idx = i;
@@ -291,7 +293,7 @@ __device__ __forceinline__ void adj_tile(const bcell_args &a, const btile_k &til
         const mc_cell cell = cells[c];
         bool coll;
         double di;
-        const double dw = mc_event(p, cell, L.dx, K, coll, di);
+        const double dw = mc_event<true>(p, cell, L.dx, K, coll, di);
         ++ev;
         if (!HAVE_SCORE && phase == 0) {
           T = T + g[c] * dw;
@@ -312,6 +314,7 @@ __device__ __forceinline__ void adj_tile(const bcell_args &a, const btile_k &til
           p.seed = a.bank.seed[idx];
           p.x = a.bank.x[idx];
           p.mu = a.bank.mu[idx];
+          mc_refresh_rcp(p);
           p.wmc = a.bank.wmc[idx];
           p.index = a.bank.index[idx];
           ev = 0;

@@ -94,6 +94,7 @@ __device__ __forceinline__ void batch_tile(const batch_args &a, const tile_k &ti
             p.seed = a.soa.seed[i];
             p.x = a.soa.x[i];
             p.mu = a.soa.mu[i];
+            mc_refresh_rcp(p);
             p.wmc = a.soa.wmc[i];
             p.index = a.soa.index[i];
             idx = i;
@@ -112,7 +113,7 @@ __device__ __forceinline__ void batch_tile(const batch_args &a, const tile_k &ti
         const int c = p.index - L.index_start;
         bool coll;
         double di;
-        const double dw = mc_event(p, cells[c], L.dx, K, coll, di);
+        const double dw = mc_event<true>(p, cells[c], L.dx, K, coll, di);
         ++ev;
         col += coll ? 1u : 0u;
         mc_tally(t_lo, t_hi, a.flags + sl.cell_off, &a.ctr->status, c, dw);

@@ -77,6 +77,7 @@ __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refil
           const int64_t i = (int64_t)(base + (unsigned long long)mc_mbcnt64(need));
           if (i < a.n) {
             load_state(a, i, p);
+            mc_refresh_rcp(p);
             idx = i;
             has = true;
             ev = 0;
@@ -95,7 +96,7 @@ __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refil
         const mc_cell cell = cells[c];
         bool coll;
         double di;
-        const double dw = mc_event(p, cell, a.L.dx, K, coll, di);
+        const double dw = mc_event<true>(p, cell, a.L.dx, K, coll, di);
         ++ev;
         col += coll ? 1u : 0u;
         mc_tally(t_lo, t_hi, a.flags, &a.ctr->status, c, dw);

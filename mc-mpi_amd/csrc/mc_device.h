@@ -77,7 +77,29 @@ struct mc_state {
   double mu;
   double wmc;
   int32_t index;
+  double inv_mu; /* RN(1/mu), kept by the kernels that divide by reciprocal */
 };
+
+/* a / b correctly rounded from y = RN(1/b) (an IEEE division):
+   q0 = RN(a y) is within 1.5 ulp of a/b; q1 = RN(q0 + r0 y) with the exact
+   remainder r0 = a - b q0 is then within half an ulp plus 2^-104 relative,
+   hence within 1 ulp; and with r1 = a - b q1 exact, RN(q1 + r1 y) is the
+   correctly rounded quotient by Markstein's theorem (y within half an ulp of
+   1/b, q1 within 1 ulp of a/b).  For |b| >= 2^-60 the fmas stay in the
+   normal range while 2^-900 <= |q0| <= 2^900; otherwise (a = 0 among them)
+   the IEEE division is used.  (mc_event uses the quotient only for
+   |mu| > EPS_PRECISION, far above 2^-60.) */
+__device__ __forceinline__ double mc_div_by_rcp(double a, double b, double y) {
+  const double q0 = a * y;
+  const double aq = __builtin_fabs(q0);
+  if (!(aq >= 0x1p-900 && aq <= 0x1p900)) return a / b;
+  const double r0 = __builtin_fma(-b, q0, a);
+  const double q1 = __builtin_fma(r0, y, q0);
+  const double r1 = __builtin_fma(-b, q1, a);
+  return __builtin_fma(r1, y, q1);
+}
+
+__device__ __forceinline__ void mc_refresh_rcp(mc_state &p) { p.inv_mu = 1.0 / p.mu; }
 
 /* Layer constants read by the step (include/layer/layer.hpp:465-485) */
 struct mc_layer_k {
@@ -116,6 +138,10 @@ __device__ __forceinline__ mc_coef mc_coef_live() {
    a special h) are the correctly rounded log and the division formed and
    compared exactly, as the reference does.  Results are identical either
    way. */
+/* RCP: the edge distance divides by the particle's stored RN(1/mu)
+   (mc_div_by_rcp; same result), refreshed when mu changes, for kernels that
+   keep the state in registers. */
+template <bool RCP = false>
 __device__ __forceinline__ double mc_event(mc_state &p, const mc_cell cell, const double dx,
                                            const mc_coef &K, bool &collided, double &di_out) {
   /* The draw and the log's table row come first; the edge quotient, which
@@ -129,15 +155,16 @@ __device__ __forceinline__ double mc_event(mc_state &p, const mc_cell cell, cons
   const bool back = p.mu < 0;
   int index_new = back ? p.index - 1 : p.index + 1;
   const double x_new_edge = (back ? p.index : p.index + 1) * dx;
-  const double q_edge = (x_new_edge - p.x) / p.mu;
+  const double q_edge = RCP ? mc_div_by_rcp(x_new_edge - p.x, p.mu, p.inv_mu)
+                            : (x_new_edge - p.x) / p.mu;
   asm volatile("" ::"v"(q_edge));
   const double di_edge = (p.mu < -MC_EPS || MC_EPS < p.mu) ? q_edge : MC_MAXREAL;
 
   const double nlog_a = -mc_log_approx(&lrow, &K);
   const double reach = cell.sig_i * di_edge;
   /* bitwise, not short-circuit: no branch around the approximate log */
-  const bool sure_cross = !mc_log_is_special(h) &
-                          (!(cell.sig_i > MC_EPS) | (nlog_a > mc_fma(reach, 0x1p-40, reach)));
+  const int sure_cross = (int)!mc_log_is_special(h) &
+                         ((int)!(cell.sig_i > MC_EPS) | (int)(nlog_a > mc_fma(reach, 0x1p-40, reach)));
   double di = di_edge;
   collided = false;
   if (!sure_cross) {
@@ -150,6 +177,7 @@ __device__ __forceinline__ double mc_event(mc_state &p, const mc_cell cell, cons
     index_new = p.index;
     p.x += di * p.mu;
     p.mu = 2 * mc_rnd_real(p.seed) - 1;
+    if (RCP) mc_refresh_rcp(p);
   } else {
     p.x = x_new_edge;
   }
