@@ -137,6 +137,25 @@ MC_FN mc_coef mc_coef_default(void) {
   return K;
 }
 
+/* a / b correctly rounded from y = RN(1/b) (an IEEE division):
+   q0 = RN(a y) is within 1.5 ulp of a/b; q1 = RN(q0 + r0 y) with the exact
+   remainder r0 = a - b q0 is then within half an ulp plus 2^-104 relative,
+   hence within 1 ulp; and with r1 = a - b q1 exact, RN(q1 + r1 y) is the
+   correctly rounded quotient by Markstein's theorem (y within half an ulp of
+   1/b, q1 within 1 ulp of a/b).  For |b| >= 2^-60 the fmas stay in the
+   normal range while 2^-900 <= |q0| <= 2^900; otherwise (a = 0 among them)
+   the IEEE division is used.  (mc_event uses the quotient only for
+   |mu| > EPS_PRECISION, far above 2^-60.) */
+MC_FN double mc_div_by_rcp(double a, double b, double y) {
+  const double q0 = a * y;
+  const double aq = __builtin_fabs(q0);
+  if (!(aq >= 0x1p-900 && aq <= 0x1p900)) return a / b;
+  const double r0 = mc_fma(-b, q0, a);
+  const double q1 = mc_fma(r0, y, q0);
+  const double r1 = mc_fma(-b, q1, a);
+  return mc_fma(r1, y, q1);
+}
+
 /* ------------------------------------------------------------------- log */
 
 /* log(x) as double-double, all steps in double-double (about 2^-100 rel). */

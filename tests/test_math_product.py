@@ -159,3 +159,47 @@ def test_log_approx_error_bound(tmp_path, oracle):
     rel = np.abs(y - ref) / np.abs(ref)
     print("max rel error 2^%.2f" % np.log2(rel.max()))
     assert rel.max() < 2.0 ** -48, np.log2(rel.max())
+
+
+DIV_SHIM = r"""
+#include <stdint.h>
+#include "mc_math.h"
+void pm_div_by_rcp(const double *a, const double *b, long n, double *q) {
+  for (long i = 0; i < n; ++i) q[i] = mc_div_by_rcp(a[i], b[i], 1.0 / b[i]);
+}
+"""
+
+
+def test_div_by_reciprocal_equals_ieee_division(tmp_path):
+    """The edge distance (x_new_edge - x) / mu is formed from a stored
+    RN(1/mu) with two fma corrections; it must equal the IEEE quotient bit
+    for bit, including divisors with all-ones significands, operands next to
+    powers of two and the fallback range (zero numerators, extreme ratios)."""
+    src = tmp_path / "pd.c"
+    src.write_text(DIV_SHIM)
+    so = tmp_path / "libpd.so"
+    subprocess.run(["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-ffp-contract=off",
+                    "-I", os.path.join(ROOT, "mc-mpi_amd", "csrc"), "-o", str(so), str(src), "-lm"],
+                   check=True)
+    L = C.CDLL(str(so))
+    L.pm_div_by_rcp.argtypes = [C.c_void_p, C.c_void_p, C.c_long, C.c_void_p]
+    rng = np.random.default_rng(10)
+    n = 400_000
+    a = [rng.random(n) * 2 - 1,
+         np.ldexp(1 + rng.random(n), rng.integers(-100, 100, n)),
+         (rng.integers(-10000, 10001, n) * 1e-4 - rng.random(n)),
+         (np.uint64(0x3FF0000000000000) + rng.integers(0, 1 << 20, n, dtype=np.uint64)).view(np.float64),
+         np.concatenate([np.zeros(1000), np.ldexp(1.0, rng.integers(-1000, 1000, n - 1000))])]
+    b = [rng.random(n) * 2 - 1,
+         np.ldexp(1 + rng.random(n), rng.integers(-60, 60, n)),
+         np.sqrt(rng.random(n)),
+         (np.uint64(0x3FEFFFFFFFFFFFFF) - rng.integers(0, 1 << 16, n, dtype=np.uint64)).view(np.float64),
+         np.ldexp(1 + rng.random(n), rng.integers(-60, 60, n))]
+    a = np.concatenate(a)
+    b = np.concatenate(b) * np.where(rng.random(5 * n) < 0.5, -1.0, 1.0)
+    keep = np.abs(b) >= 2.0 ** -60
+    a, b = np.ascontiguousarray(a[keep]), np.ascontiguousarray(b[keep])
+    q = np.empty_like(a)
+    L.pm_div_by_rcp(a.ctypes.data, b.ctypes.data, len(a), q.ctypes.data)
+    with np.errstate(all="ignore"):
+        assert np.all(_same(q, a / b))
