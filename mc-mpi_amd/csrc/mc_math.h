@@ -433,9 +433,9 @@ MC_FN_COLD double mc_exp_special(double x) {
 
 MC_FN double mc_exp_k(double x, const mc_coef *K) {
   if (!(x > -708.3 && x < 709.7)) return mc_exp_special(x);
-  /* |x| < 2^-54: exp(x) rounds to 1 (1 + x lies strictly inside the
-     rounding interval of 1, and so does exp(x) = 1 + x + O(x^2)). */
-  if (__builtin_fabs(x) < 0x1p-54) return 1.0;
+  /* Tiny |x| (zero and subnormals included) need no test of their own:
+     kd = 0, r = x, and the fast path's error stays relative to exp(x) ~ 1
+     (a failed rounding test goes to mc_exp_tiny below). */
   double kd = __builtin_rint(x * MC_INV_LN2_128);
   int k = (int)kd;
   /* r = x - kd*ln2/128 as rh + rl, |kd| < 2^17, ln2/128 = A + B (+ below
@@ -468,8 +468,9 @@ MC_FN double mc_exp_k(double x, const mc_coef *K) {
   double yl = mc_fma(th, eh, -yh);
   yl = yl + mc_fma(th, el, tl * eh);
   mc_fast_two_sum(yh, yl, &yh, &yl);
-  double sc = mc_pow2((k - j) / 128);
-  if (mc_fma(yl, K->ziv, yh) == yh) return yh * sc; /* mc_rounds_to_hi */
+  /* times 2^((k - j)/128) = 2^(k >> 7): exact, the result being normal for
+     -708.3 < x < 709.7 */
+  if (mc_fma(yl, K->ziv, yh) == yh) return __builtin_ldexp(yh, k >> 7); /* mc_rounds_to_hi */
   if (__builtin_fabs(x) < 0x1p-30) return mc_exp_tiny(x);
   int kk;
   mc_exp_dd(x, 0.0, &yh, &yl, &kk);
