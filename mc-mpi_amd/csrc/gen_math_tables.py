@@ -173,10 +173,10 @@ def log_table():
     return float(rmax_all), float(err_all), rows
 
 
-def exp_table():
+def exp_table(n):
     rows = []
-    for j in range(128):
-        v = Decimal(2) ** (Decimal(j) / Decimal(128))
+    for j in range(n):
+        v = Decimal(2) ** (Decimal(j) / Decimal(n))
         rows.append(dd(v))
     return rows
 
@@ -216,9 +216,15 @@ def main():
         w(f"  {{{hexf(c)}, {hexf(hi)}, {hexf(mid)}, {hexf(lo)}}}, /* {i} */")
     w("};")
     w("")
-    w("/* {hi, lo} of 2^(j/128) */")
+    w("/* {hi, lo} of 2^(j/128) (accurate path) */")
     w("MC_TAB double mc_exp_tab[128][2] = {")
-    for j, (hi, lo) in enumerate(exp_table()):
+    for j, (hi, lo) in enumerate(exp_table(128)):
+        w(f"  {{{hexf(hi)}, {hexf(lo)}}}, /* {j} */")
+    w("};")
+    w("")
+    w("/* {hi, lo} of 2^(j/1024) (fast path) */")
+    w("MC_TAB double mc_exp_tab10[1024][2] = {")
+    for j, (hi, lo) in enumerate(exp_table(1024)):
         w(f"  {{{hexf(hi)}, {hexf(lo)}}}, /* {j} */")
     w("};")
     w("")
@@ -244,6 +250,18 @@ def main():
     assert to_nbits(Fraction(a), 36) == Fraction(a)
     w(f"#define MC_LN2_128_A {hexf(a)}")
     w(f"#define MC_LN2_128_B {hexf(b)}")
+    # ln2/1024 likewise for the fast path of exp: A has 33 significant bits,
+    # so k*A is exact for |k| < 2^20 (|x| < 709.7 gives |k| <= 1048464);
+    # |B| <= 2^-44 and |ln2/1024 - A - B| <= 2^-97, so |k| times that
+    # residual stays below 2^-77.
+    a, b, _ = split_nbits(ln2 / 1024, 33)
+    assert Decimal(709.7) * 1024 / ln2 < 2 ** 20
+    assert abs(Decimal(b)) <= Decimal(2) ** -44
+    assert abs(ln2 / 1024 - Decimal(a) - Decimal(b)) <= Decimal(2) ** -97
+    assert to_nbits(Fraction(a), 33) == Fraction(a)
+    w(f"#define MC_LN2_1024_A {hexf(a)}")
+    w(f"#define MC_LN2_1024_B {hexf(b)}")
+    w(f"#define MC_INV_LN2_1024 {hexf(float(Decimal(1024) / ln2))}")
     w("")
     # 1/n (n = 1..16) and 1/n! (n = 0..16) as double-double
     w("/* {hi, lo} of 1/n, n = 0..16 (entry 0 unused) */")

@@ -102,7 +102,7 @@ struct mc_layer_k {
 __device__ __forceinline__ mc_coef mc_coef_live() {
   mc_coef K = mc_coef_default();
   asm volatile("" : "+v"(K.l8), "+v"(K.l7), "+v"(K.l6), "+v"(K.l5), "+v"(K.l4), "+v"(K.l3));
-  asm volatile("" : "+v"(K.e6), "+v"(K.e5), "+v"(K.e4), "+v"(K.e3), "+v"(K.ziv));
+  asm volatile("" : "+v"(K.e5), "+v"(K.e4), "+v"(K.e3), "+v"(K.ziv));
   asm volatile("" : "+v"(K.a6), "+v"(K.a5), "+v"(K.a3));
   return K;
 }

@@ -22,9 +22,9 @@
  *     (relative error about 2^-100).  The fast log is below 2^-68.6: a bound
  *     gen_math_tables.py computes for every table row and asserts; the
  *     largest rows are next to x = 1, where log x is small.  The fast exp is
- *     below 2^-71.8 (series truncation below 2^-72, argument reduction below
- *     2^-79, roundings below 2^-76).  An fma in place of a multiply and an
- *     add only removes a rounding.  The result can differ from the correctly
+ *     below 2^-72 (a 1,024-entry table, argument reduction below 2^-76,
+ *     each rounding below 2^-75.5, series truncation below 2^-78).  An fma
+ *     in place of a multiply and an add only removes a rounding.  The result can differ from the correctly
  *     rounded one only when the exact value lies within about 2^-100
  *     (relative) of a rounding boundary.
  *
@@ -101,7 +101,7 @@ MC_FN int mc_round_ok(double hi, double lo, double err) {
 
 /* The same certificate in one fma, for hi + lo (hi = RN(hi + lo)) within
    relative error eps of the exact value Y, eps < 2^-68 (the fast paths are
-   below 2^-68.6 for log and 2^-71.8 for exp).  Let b be the distance from hi to its rounding boundary on
+   below 2^-68.6 for log and 2^-72 for exp).  Let b be the distance from hi to its rounding boundary on
    the side of lo: b >= 2^-54 |hi| (half an ulp of hi, or a quarter below a
    power of two).  RN(hi + e lo) == hi with e = 1 + 2^-13 (one rounding, at
    that boundary, ties included) gives |lo| <= b/e, so on lo's side
@@ -121,7 +121,7 @@ MC_FN int mc_rounds_to_hi(double hi, double lo) { return mc_fma(lo, MC_ZIV_E, hi
    either way. */
 typedef struct {
   double l8, l7, l6, l5, l4, l3; /* -1/8, 1/7, -1/6, 1/5, -1/4, 1/3 */
-  double e6, e5, e4, e3;         /* 1/6!, 1/5!, 1/4!, 1/3! */
+  double e5, e4, e3;             /* 1/5!, 1/4!, 1/3! */
   double ziv;                    /* MC_ZIV_E */
   double a6, a5, a3;             /* mc_log_approx: -1/6, 1/5, 1/3 */
 } mc_coef;
@@ -130,7 +130,7 @@ MC_FN mc_coef mc_coef_default(void) {
   mc_coef K;
   K.l8 = -mc_inv_n[8][0]; K.l7 = mc_inv_n[7][0]; K.l6 = -mc_inv_n[6][0];
   K.l5 = mc_inv_n[5][0];  K.l4 = -mc_inv_n[4][0]; K.l3 = mc_inv_n[3][0];
-  K.e6 = mc_inv_fact[6][0]; K.e5 = mc_inv_fact[5][0];
+  K.e5 = mc_inv_fact[5][0];
   K.e4 = mc_inv_fact[4][0]; K.e3 = mc_inv_fact[3][0];
   K.ziv = MC_ZIV_E;
   K.a6 = -1.0 / 6.0; K.a5 = 0.2; K.a3 = 1.0 / 3.0;
@@ -436,41 +436,40 @@ MC_FN double mc_exp_k(double x, const mc_coef *K) {
   /* Tiny |x| (zero and subnormals included) need no test of their own:
      kd = 0, r = x, and the fast path's error stays relative to exp(x) ~ 1
      (a failed rounding test goes to mc_exp_tiny below). */
-  double kd = __builtin_rint(x * MC_INV_LN2_128);
-  int k = (int)kd;
-  /* r = x - kd*ln2/128 as rh + rl, |kd| < 2^17, ln2/128 = A + B (+ below
-     2^-97).  t = x - kd*A is exact (kd*A has 53 bits; t is a multiple of
-     min(ulp(x), 2^-43) below 2^-8 in magnitude).  rh = RN(t - kd*B);
-     t - rh = kd*B - (rounding of rh) is below 2^-29 + 2^-62, so its
-     rounding errs by less than 2^-81; rl = RN((t - rh) - kd*B) then leaves
-     |rh + rl - r| < 2^-79. */
-  const double t = mc_fma(-kd, MC_LN2_128_A, x);
-  const double rh = mc_fma(-kd, MC_LN2_128_B, t);
-  const double rl = mc_fma(-kd, MC_LN2_128_B, t - rh);
-  /* exp(r) - 1 = r + r^2/2 + r^3 * Q(r), Q = 1/6 + r/24 + r^2/120 + r^3/720;
-     the omitted tail r^7/5040 + ... is below 2^-72 for |r| < 2^-8.5 */
-  double r2h = rh * rh;
-  double r2l = mc_fma(rh, rh, -r2h);
-  double q = mc_fma(rh, K->e6, K->e5);
-  q = mc_fma(rh, q, K->e4);
-  q = mc_fma(rh, q, K->e3);
-  q = (r2h * rh) * q;
-  double ah, al;
-  mc_fast_two_sum(rh, 0.5 * r2h, &ah, &al);
-  al = al + (rl + mc_fma(0.5, r2l, mc_fma(rh, rl, q)));
-  /* 1 + a */
+  const double kd = __builtin_rint(x * MC_INV_LN2_1024);
+  const int k = (int)kd;
+  /* r = x - kd*ln2/1024 as rh + rl, |kd| < 2^20, ln2/1024 = A + B (+ below
+     2^-97), |r| < 2^-11.5.  t = x - kd*A is exact (kd*A has at most 53 bits;
+     t is a multiple of min(ulp(x), 2^-43) below 2^-11 in magnitude).
+     rh = RN(t - kd*B); t - rh = kd*B - (rounding of rh) is below 2^-24 +
+     2^-65, so its rounding errs by less than 2^-77; rl = RN((t - rh) - kd*B)
+     then leaves |rh + rl - r| < 2^-76. */
+  const double t = mc_fma(-kd, MC_LN2_1024_A, x);
+  const double rh = mc_fma(-kd, MC_LN2_1024_B, t);
+  const double rl = mc_fma(-kd, MC_LN2_1024_B, t - rh);
+  /* exp(r) = 1 + r + b, b = r^2 (1/2 + r/6 + r^2/24 + r^3/120), |b| < 2^-24,
+     evaluated in plain double; the head 1 + rh by FastTwoSum.  Errors,
+     relative to the result: series tail r^6/720 < 2^-78.7; the roundings
+     of b, of rl + b and of el below 2^-76.5 each; rl * (exp(rh) - 1)
+     dropped, below 2^-76.5; the product by 2^(j/1024) = th + tl (fma
+     remainders, th * el rounded) below 2^-75.5; the reduction below 2^-76.
+     In all below 2^-72. */
+  double c = mc_fma(rh, K->e5, K->e4);
+  c = mc_fma(rh, c, K->e3);
+  c = mc_fma(rh, c, 0.5);
+  const double b = (rh * rh) * c;
   double eh, el;
-  mc_fast_two_sum(1.0, ah, &eh, &el);
-  el = el + al;
-  int j = k & 127;
-  double th = mc_exp_tab[j][0], tl = mc_exp_tab[j][1];
+  mc_fast_two_sum(1.0, rh, &eh, &el);
+  el = el + (rl + b);
+  const int j = k & 1023;
+  const double th = mc_exp_tab10[j][0], tl = mc_exp_tab10[j][1];
   double yh = th * eh;
   double yl = mc_fma(th, eh, -yh);
   yl = yl + mc_fma(th, el, tl * eh);
   mc_fast_two_sum(yh, yl, &yh, &yl);
-  /* times 2^((k - j)/128) = 2^(k >> 7): exact, the result being normal for
+  /* times 2^((k - j)/1024) = 2^(k >> 10): exact, the result being normal for
      -708.3 < x < 709.7 */
-  if (mc_fma(yl, K->ziv, yh) == yh) return __builtin_ldexp(yh, k >> 7); /* mc_rounds_to_hi */
+  if (mc_fma(yl, K->ziv, yh) == yh) return __builtin_ldexp(yh, k >> 10); /* mc_rounds_to_hi */
   if (__builtin_fabs(x) < 0x1p-30) return mc_exp_tiny(x);
   int kk;
   mc_exp_dd(x, 0.0, &yh, &yl, &kk);

@@ -93,13 +93,16 @@ def test_exp_full_range(pm, oracle):
 
 
 def test_exp_near_multiples_of_ln2_over_128(pm, oracle):
-    """Arguments next to k*ln2/128, where the reduced argument r nearly
-    cancels and the two-fma reduction's error bound is tightest."""
+    """Arguments next to k*ln2/128 and k*ln2/1024 (the accurate path's and the
+    fast path's reduction steps), where the reduced argument r nearly cancels
+    and the two-fma reduction's error bound is tightest."""
     rng = np.random.default_rng(7)
     k = rng.integers(-130_000, 131_000, 300_000).astype(np.float64)
     off = (rng.integers(-1024, 1024, 300_000).astype(np.float64)
            * 2.0 ** -rng.integers(40, 70, 300_000).astype(np.float64))
-    x = k * (np.log(2.0) / 128.0) + off
+    k10 = rng.integers(-1_040_000, 1_048_000, 300_000).astype(np.float64)
+    x = np.concatenate([k * (np.log(2.0) / 128.0), k10 * (np.log(2.0) / 1024.0)]) \
+        + np.concatenate([off, off])
     x = x[(x > -708.3) & (x < 709.7)]
     assert np.all(_same(pm("exp", x), oracle.math_eval("exp", x)))
 
