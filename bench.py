@@ -45,11 +45,11 @@ BYTES_PER_EVENT = 72  # algorithmic: the 36-byte particle state read + written p
 # HBM bytes per event measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
 # this workload (1e7 particles), scaled by the copy-kernel calibration (read
 # x2.0, write x1.0); the counter runs are summarised under profiles/.
-MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0613, "event": 88.3}
+MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0619, "event": 88.5}
 # VALU wave-instructions per event (rocprofv3 --pmc SQ_INSTS_VALU, same runs).
 # gfx950 issues one wave64 VALU instruction per SIMD every 4 cycles:
 # 256 CUs x 4 SIMDs x 2.4 GHz / 4 = 6.144e11 wave-instructions/s.
-MEASURED_VALU_PER_EVENT = {"history": 2.879, "event": 4.542}
+MEASURED_VALU_PER_EVENT = {"history": 2.236, "event": 3.921}
 VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
 
 
@@ -456,8 +456,10 @@ def main():
                      "traffic": None if traffic is None else traffic * ev_step,
                      "note": ("72 algorithmic B/event (36-byte state read + written, the "
                               "event formulation); the history kernel keeps the state in "
-                              "registers (measured 0.06 B/event) and is bound by FP64 VALU "
-                              "issue (see valu), hence frac > 1") if args.kernel == "history" else
+                              "registers (measured 0.06 B/event), hence frac > 1; it is "
+                              "bound by instruction issue and latency (see valu; 4 waves "
+                              "per SIMD, the LDS tally of 10,000 cells filling the CU)") \
+                             if args.kernel == "history" else
                              "72 algorithmic B/event; measured 88 B/event (44-byte pool slots)",
                      "valu": valu_rate(args.kernel, ev_step, k_avg_s)},
         "cpu_baseline": None,
