@@ -438,6 +438,13 @@ MC_FN double mc_exp_k(double x, const mc_coef *K) {
      (a failed rounding test goes to mc_exp_tiny below). */
   const double kd = __builtin_rint(x * MC_INV_LN2_1024);
   const int k = (int)kd;
+  const int j = k & 1023;
+  const double th = mc_exp_tab10[j][0], tl = mc_exp_tab10[j][1];
+#if defined(__HIP_DEVICE_COMPILE__)
+  /* keep the table row's load ahead of the series below (the scheduler
+     would otherwise issue it just before its use and wait for it there) */
+  __builtin_amdgcn_sched_barrier(0);
+#endif
   /* r = x - kd*ln2/1024 as rh + rl, |kd| < 2^20, ln2/1024 = A + B (+ below
      2^-97), |r| < 2^-11.5.  t = x - kd*A is exact (kd*A has at most 53 bits;
      t is a multiple of min(ulp(x), 2^-43) below 2^-11 in magnitude).
@@ -461,8 +468,6 @@ MC_FN double mc_exp_k(double x, const mc_coef *K) {
   double eh, el;
   mc_fast_two_sum(1.0, rh, &eh, &el);
   el = el + (rl + b);
-  const int j = k & 1023;
-  const double th = mc_exp_tab10[j][0], tl = mc_exp_tab10[j][1];
   double yh = th * eh;
   double yl = mc_fma(th, eh, -yh);
   yl = yl + mc_fma(th, el, tl * eh);
