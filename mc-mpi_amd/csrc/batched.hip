@@ -113,7 +113,7 @@ __device__ __forceinline__ void batch_tile(const batch_args &a, const tile_k &ti
         const int c = p.index - L.index_start;
         bool coll;
         double di;
-        const double dw = mc_event<true>(p, cells[c], L.dx, K, coll, di);
+        const double dw = mc_event<true>(p, cells, c, L.dx, K, coll, di);
         ++ev;
         col += coll ? 1u : 0u;
         mc_tally(t_lo, t_hi, a.flags + sl.cell_off, &a.ctr->status, c, dw);

@@ -208,7 +208,7 @@ __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned ch
           const int c = p.index - a.L.index_start;
           bool coll;
           double di;
-          const double dw = mc_event(p, cells[c], a.L.dx, K, coll, di);
+          const double dw = mc_event(p, cells, c, a.L.dx, K, coll, di);
           ++ev;
           ++pass_ev;
           col += coll ? 1u : 0u;

@@ -93,10 +93,9 @@ __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refil
       for (int k = 0; k < refill_k; ++k) {
         if (!mc_alive(p, a.L) || ev >= a.max_events) break;
         const int c = p.index - a.L.index_start;
-        const mc_cell cell = cells[c];
         bool coll;
         double di;
-        const double dw = mc_event<true>(p, cell, a.L.dx, K, coll, di);
+        const double dw = mc_event<true>(p, cells, c, a.L.dx, K, coll, di);
         ++ev;
         col += coll ? 1u : 0u;
         mc_tally(t_lo, t_hi, a.flags, &a.ctr->status, c, dw);
