@@ -314,11 +314,10 @@ MC_FN int mc_log_is_special(double x) {
   return (uint32_t)(mc_asu64(x) >> 32) - 0x00100000u >= 0x7FF00000u - 0x00100000u;
 }
 
-/* The table row of x, or of 1.0 in place of a special x (no branch: the
-   special result replaces the value in mc_log_finish). */
-MC_FN mc_log_row mc_log_start(double x) {
-  return mc_log_lookup(mc_log_is_special(x) ? 0x3FF0000000000000ULL : mc_asu64(x), 0);
-}
+/* The table row of x.  For a special x the row is meaningless but valid
+   (the index is a 9-bit field of the bits); mc_log_finish replaces the value,
+   and the transport step sends such x to its exact path. */
+MC_FN mc_log_row mc_log_start(double x) { return mc_log_lookup(mc_asu64(x), 0); }
 
 MC_FN double mc_log_finish(double x, const mc_log_row *R, const mc_coef *K) {
   const double y = mc_log_eval(R, K);
@@ -402,7 +401,7 @@ MC_FN_COLD double mc_exp_special(double x) {
   double yh, yl;
   int k;
   if (x > 0.0) {
-    /* 709.7 .. 709.78: result near DBL_MAX; scale in two steps */
+    /* 708.3 .. 709.78: result near DBL_MAX; scale in two steps */
     mc_exp_dd(x, 0.0, &yh, &yl, &k);
     double y = yh + yl;
     return (y * mc_pow2(k - 1)) * 2.0;
@@ -432,7 +431,9 @@ MC_FN_COLD double mc_exp_special(double x) {
 }
 
 MC_FN double mc_exp_k(double x, const mc_coef *K) {
-  if (!(x > -708.3 && x < 709.7)) return mc_exp_special(x);
+  /* |x| >= 708.3 (and NaN) out of line; the band 708.3 .. 709.7 is exact
+     there too */
+  if (!(__builtin_fabs(x) < 708.3)) return mc_exp_special(x);
   /* Tiny |x| (zero and subnormals included) need no test of their own:
      kd = 0, r = x, and the fast path's error stays relative to exp(x) ~ 1
      (a failed rounding test goes to mc_exp_tiny below). */
@@ -473,7 +474,7 @@ MC_FN double mc_exp_k(double x, const mc_coef *K) {
   yl = yl + mc_fma(th, el, tl * eh);
   mc_fast_two_sum(yh, yl, &yh, &yl);
   /* times 2^((k - j)/1024) = 2^(k >> 10): exact, the result being normal for
-     -708.3 < x < 709.7 */
+     |x| < 708.3 */
   if (mc_fma(yl, K->ziv, yh) == yh) return __builtin_ldexp(yh, k >> 10); /* mc_rounds_to_hi */
   if (__builtin_fabs(x) < 0x1p-30) return mc_exp_tiny(x);
   int kk;
