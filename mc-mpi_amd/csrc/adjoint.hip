@@ -112,10 +112,10 @@ __global__ __launch_bounds__(1024) void k_adjoint(adj_args a, int refill_k) {
       for (int k = 0; k < refill_k; ++k) {
         if (!mc_alive(p, a.L) || ev >= max_ev) break;
         const int c = p.index - a.L.index_start;
-        const mc_cell cell = cells[c];
+        mc_cell cell;
         bool coll;
         double di;
-        const double dw = mc_event<true>(p, cell, a.L.dx, K, coll, di);
+        const double dw = mc_event<true>(p, cells, c, a.L.dx, K, coll, di, &cell);
         ++ev;
         if (!HAVE_SCORE && phase == 0) {
           T = T + g[c] * dw;

@@ -290,10 +290,10 @@ __device__ __forceinline__ void adj_tile(const bcell_args &a, const btile_k &til
       for (int k = 0; k < refill_k; ++k) {
         if (!mc_alive(p, L) || ev >= max_ev) break;
         const int c = p.index - L.index_start;
-        const mc_cell cell = cells[c];
+        mc_cell cell;
         bool coll;
         double di;
-        const double dw = mc_event<true>(p, cell, L.dx, K, coll, di);
+        const double dw = mc_event<true>(p, cells, c, L.dx, K, coll, di, &cell);
         ++ev;
         if (!HAVE_SCORE && phase == 0) {
           T = T + g[c] * dw;

@@ -122,11 +122,10 @@ __device__ __forceinline__ mc_coef mc_coef_live() {
 /* RCP: the edge distance divides by the particle's stored RN(1/mu)
    (mc_div_by_rcp; same result), refreshed when mu changes, for kernels that
    keep the state in registers. */
-template <bool RCP, bool ORDERED>
-__device__ __forceinline__ double mc_event_core(mc_state &p, const mc_cell *cells, int c,
-                                                const mc_cell given, const double dx,
-                                                const mc_coef &K, bool &collided,
-                                                double &di_out) {
+template <bool RCP = false>
+__device__ __forceinline__ double mc_event(mc_state &p, const mc_cell *cells, int c,
+                                           const double dx, const mc_coef &K, bool &collided,
+                                           double &di_out, mc_cell *cell_out = nullptr) {
   /* The draw and the log's table row come first; the edge quotient, which
      does not depend on them, is formed while the row's loads are in flight
      (the empty asm keeps it there instead of letting the compiler sink it
@@ -134,16 +133,15 @@ __device__ __forceinline__ double mc_event_core(mc_state &p, const mc_cell *cell
      |mu| <= EPS the discarded quotient may be inf or NaN and is never used. */
   const double h = mc_rnd_real(p.seed);
   const mc_log_row lrow = mc_log_start(h);
-  /* ORDERED: cells[c] is read after the row.  Waits for loads are in
+  /* The cell, cells[c], is read after the row.  Waits for loads are in
      order, and the wait for the row (an L1 hit) must not wait for the cell
      (from L2 when the cells are in global memory).  The empty asm makes the
-     index opaque there, so the compiler cannot hoist the read above it. */
-  mc_cell cell = given;
-  if (ORDERED) {
-    __builtin_amdgcn_sched_barrier(0);
-    asm volatile("" : "+v"(c));
-    cell = cells[c];
-  }
+     index opaque there, so the compiler cannot hoist the read above it.
+     *cell_out (if given) receives the cell. */
+  __builtin_amdgcn_sched_barrier(0);
+  asm volatile("" : "+v"(c));
+  const mc_cell cell = cells[c];
+  if (cell_out) *cell_out = cell;
 
   const bool back = p.mu < 0;
   int index_new = back ? p.index - 1 : p.index + 1;
@@ -180,21 +178,6 @@ __device__ __forceinline__ double mc_event_core(mc_state &p, const mc_cell *cell
   p.index = index_new;
   di_out = di;
   return dw;
-}
-
-/* the step on a cell passed by value */
-template <bool RCP = false>
-__device__ __forceinline__ double mc_event(mc_state &p, const mc_cell cell, const double dx,
-                                           const mc_coef &K, bool &collided, double &di_out) {
-  return mc_event_core<RCP, false>(p, nullptr, 0, cell, dx, K, collided, di_out);
-}
-
-/* the step on cells[c], read after the log's table row */
-template <bool RCP = false>
-__device__ __forceinline__ double mc_event(mc_state &p, const mc_cell *cells, int c,
-                                           const double dx, const mc_coef &K, bool &collided,
-                                           double &di_out) {
-  return mc_event_core<RCP, true>(p, cells, c, mc_cell{0.0, 0.0}, dx, K, collided, di_out);
 }
 
 /* src/layer.cpp:195-197: the history continues while this holds */
