@@ -90,10 +90,7 @@ struct mc_layer_k {
   double min_weight;
 };
 
-/* One transport event: src/layer.cpp:123-190 (Layer::particle_step) in FP64,
-   EPS_PRECISION = 1e-4, MAXREAL = DBL_MAX, log/exp correctly rounded.
-   Returns the absorbed weight dw and the local cell it is tallied to;
-   `collided` reports which branch was taken. */
+/* EPS_PRECISION and MAXREAL of include/types/types.hpp:6-22, real_t = double */
 #define MC_EPS 1e-4
 #define MC_MAXREAL 1.7976931348623157e308
 
@@ -107,10 +104,13 @@ __device__ __forceinline__ mc_coef mc_coef_live() {
   return K;
 }
 
-/* One particle_step (src/layer.cpp:129-180).  The reference draws the
-   flight length di = -log(h) / sig_i and the edge distance di_edge, and the
-   particle collides iff di < di_edge (both correctly rounded quotients
-   here).  On fine meshes most events cross the cell edge, and then di itself
+/* One transport event: Layer::particle_step (src/layer.cpp:123-190) in FP64
+   with correctly rounded log/exp, on cell c (cells[c]).  Returns the
+   absorbed weight dw (tallied by the caller to cell c); `collided` reports
+   which branch was taken and di_out the flight length.
+   The reference draws the flight length di = -log(h) / sig_i and the edge
+   distance di_edge, and the particle collides iff di < di_edge (both
+   correctly rounded quotients here).  On fine meshes most events cross the cell edge, and then di itself
    is never used: the event first decides the comparison with a plain-double
    log (mc_log_approx, within 2^-48) and a product, with a 2^-40 margin:
      -log(h) > sig_i * di_edge * (1 + 2^-40)  (computed in floating point)
@@ -118,8 +118,8 @@ __device__ __forceinline__ mc_coef mc_coef_live() {
    particle crosses.  Only when that is not certain (a collision, a near tie,
    a special h) are the correctly rounded log and the division formed and
    compared exactly, as the reference does.  Results are identical either
-   way. */
-/* RCP: the edge distance divides by the particle's stored RN(1/mu)
+   way.
+   RCP: the edge distance divides by the particle's stored RN(1/mu)
    (mc_div_by_rcp; same result), refreshed when mu changes, for kernels that
    keep the state in registers. */
 template <bool RCP = false>
