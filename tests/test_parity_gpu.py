@@ -383,3 +383,53 @@ def test_full_size_properties_and_sampled_histories(ctx, oracle, name, n, kernel
     o = _oracle_run(oracle, wl, bank=wl.bank[idx])
     _assert_bank_equal(fin[idx], o["bank"])
     assert np.array_equal(r[idx].astype(np.int32), o["result"])
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_crossing_decision_and_edge_cases(ctx, oracle, kernel):
+    """The step decides most crossings from a plain-double log with a margin
+    and divides by a stored reciprocal of mu; both must reproduce the
+    reference's branch and quotients exactly.  Cells without interaction
+    (sigma_i <= EPS), collision-dominated cells (mean free path ~ dx/50),
+    tiny cross sections, directions at and below EPS, particles exactly on
+    the edge they move to (zero numerator) and just beyond it (negative
+    edge distance)."""
+    _say(f"crossing decision edge cases, {kernel}")
+    m = 240
+    rng = np.random.default_rng(11)
+    dx = 1.0 / m
+    kind = np.arange(m) % 4
+    sigs = np.where(kind == 0, rng.lognormal(0.0, 0.5, m),
+                    np.where(kind == 1, 50.0 / dx * rng.uniform(0.5, 1.5, m),
+                             np.where(kind == 2, 1e-6, rng.lognormal(1.0, 1.0, m))))
+    ar = np.where(kind == 0, 1.0, rng.uniform(0.05, 0.95, m))   # kind 0: sigma_i = 0
+    from mcgpu import configs
+    n = 6000
+    bank = np.zeros(n, dtype=oracle.P64)
+    bank["seed"] = configs.seed_chain(n)
+    bank["wmc"] = 1.0
+    idx = rng.integers(0, m, n)
+    bank["index"] = idx
+    frac = rng.random(n)
+    sel = rng.integers(0, 6, n)
+    mu = rng.uniform(-1.0, 1.0, n)
+    mu = np.where(sel == 1, rng.choice([1e-4, -1e-4, 1e-5, 0.0, 2e-4, -2e-4], n), mu)
+    x = (idx + frac) * dx
+    x = np.where(sel == 2, idx * dx, x)                       # on the left edge
+    x = np.where(sel == 3, (idx + 1) * dx, x)                 # on the right edge
+    x = np.where(sel == 4, np.nextafter((idx + 1) * dx, 2.0), x)  # just beyond it
+    x = np.where(sel == 5, np.nextafter(idx * dx, -1.0), x)   # just before the left edge
+    mu = np.where(sel == 2, -np.abs(mu) - 1e-3, mu)           # moving to the left edge
+    mu = np.where(sel == 3, np.abs(mu) + 1e-3, mu)            # moving to the right edge
+    bank["x"] = x
+    bank["mu"] = mu
+    o = oracle.simulate_bank(64, 0, m, dx, 1e-3, sigs, ar, bank, nthread=8)
+    b = bank.copy()
+    w = np.zeros(m)
+    import mcgpu
+    result, st, rc = ctx.simulate(b, sigs, ar, w, 0, m, dx, 1e-3,
+                                  options=mcgpu.make_options(kernel))
+    _assert_bank_equal(b, o["bank"])
+    assert np.array_equal(result, o["result"])
+    _assert_tally(w, o)
+    assert st["collisions"] > 0 and st["events"] > st["collisions"]
