@@ -161,10 +161,6 @@ def log_table():
                 if best is None or err < best[0]:
                     best = (err, rmax, cd, hi, mid, lo)
         assert best is not None, j
-        if best[3] != 0:
-            # mc_log_approx: |log m| >= |head| / 3 over the row (e = 0)
-            lmin = min(abs(math.log(float(a))), abs(math.log(float(b))))
-            assert lmin * 3 >= abs(best[3]) * (1 + 1e-9), (j, lmin, best[3])
         err_all = max(err_all, best[0])
         rmax_all = max(rmax_all, best[1])
         rows.append((j,) + best[2:])

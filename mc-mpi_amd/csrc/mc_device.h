@@ -100,7 +100,6 @@ __device__ __forceinline__ mc_coef mc_coef_live() {
   mc_coef K = mc_coef_default();
   asm volatile("" : "+v"(K.l8), "+v"(K.l7), "+v"(K.l6), "+v"(K.l5), "+v"(K.l4), "+v"(K.l3));
   asm volatile("" : "+v"(K.e5), "+v"(K.e4), "+v"(K.e3), "+v"(K.ziv));
-  asm volatile("" : "+v"(K.a6), "+v"(K.a5), "+v"(K.a3));
   return K;
 }
 
@@ -110,15 +109,15 @@ __device__ __forceinline__ mc_coef mc_coef_live() {
    which branch was taken and di_out the flight length.
    The reference draws the flight length di = -log(h) / sig_i and the edge
    distance di_edge, and the particle collides iff di < di_edge (both
-   correctly rounded quotients here).  On fine meshes most events cross the cell edge, and then di itself
-   is never used: the event first decides the comparison with a plain-double
-   log (mc_log_approx, within 2^-48) and a product, with a 2^-40 margin:
-     -log(h) > sig_i * di_edge * (1 + 2^-40)  (computed in floating point)
-   implies the exact -log(h) / sig_i > di_edge, so RN(di) >= di_edge and the
-   particle crosses.  Only when that is not certain (a collision, a near tie,
-   a special h) are the correctly rounded log and the division formed and
-   compared exactly, as the reference does.  Results are identical either
-   way.
+   correctly rounded quotients here).  On fine meshes most events cross the
+   cell edge, and then di itself is never used.  For h in [0, 1],
+   -log(h) >= 1 - h, so
+     1 - h > sig_i * di_edge * (1 + 2^-40)  (computed in floating point)
+   implies the exact -log(h) / sig_i > di_edge, hence RN(di) >= di_edge and
+   the particle crosses: such an event needs no log at all.  Only when that
+   is not certain (h near 1, a collision, a thick cell, |mu| <= EPS) are the
+   correctly rounded log and the division formed and compared exactly, as the
+   reference does.  Results are identical either way.
    RCP: the edge distance divides by the particle's stored RN(1/mu)
    (mc_div_by_rcp; same result), refreshed when mu changes, for kernels that
    keep the state in registers. */
@@ -126,40 +125,28 @@ template <bool RCP = false>
 __device__ __forceinline__ double mc_event(mc_state &p, const mc_cell *cells, int c,
                                            const double dx, const mc_coef &K, bool &collided,
                                            double &di_out, mc_cell *cell_out = nullptr) {
-  /* The draw and the log's table row come first; the edge quotient, which
-     does not depend on them, is formed while the row's loads are in flight
-     (the empty asm keeps it there instead of letting the compiler sink it
-     past the log).  It is formed unconditionally and then selected; when
-     |mu| <= EPS the discarded quotient may be inf or NaN and is never used. */
-  const double h = mc_rnd_real(p.seed);
-  const mc_log_row lrow = mc_log_start(h);
-  /* The cell, cells[c], is read after the row.  Waits for loads are in
-     order, and the wait for the row (an L1 hit) must not wait for the cell
-     (from L2 when the cells are in global memory).  The empty asm makes the
-     index opaque there, so the compiler cannot hoist the read above it.
+  /* The edge quotient is formed unconditionally and then selected; when
+     |mu| <= EPS the discarded quotient may be inf or NaN and is never used.
      *cell_out (if given) receives the cell. */
-  __builtin_amdgcn_sched_barrier(0);
-  asm volatile("" : "+v"(c));
   const mc_cell cell = cells[c];
   if (cell_out) *cell_out = cell;
+  const double h = mc_rnd_real(p.seed);
 
   const bool back = p.mu < 0;
   int index_new = back ? p.index - 1 : p.index + 1;
   const double x_new_edge = (back ? p.index : p.index + 1) * dx;
   const double q_edge = RCP ? mc_div_by_rcp(x_new_edge - p.x, p.mu, p.inv_mu)
                             : (x_new_edge - p.x) / p.mu;
-  asm volatile("" ::"v"(q_edge));
   const double di_edge = (p.mu < -MC_EPS || MC_EPS < p.mu) ? q_edge : MC_MAXREAL;
 
-  const double nlog_a = -mc_log_approx(&lrow, &K);
   const double reach = cell.sig_i * di_edge;
-  /* bitwise, not short-circuit: no branch around the approximate log */
-  const int sure_cross = (int)!mc_log_is_special(h) &
-                         ((int)!(cell.sig_i > MC_EPS) | (int)(nlog_a > mc_fma(reach, 0x1p-40, reach)));
+  /* bitwise, not short-circuit: one test, no branch */
+  const int sure_cross =
+      (int)!(cell.sig_i > MC_EPS) | (int)((1.0 - h) > mc_fma(reach, 0x1p-40, reach));
   double di = di_edge;
   collided = false;
   if (!sure_cross) {
-    const double nlog = -mc_log_finish(h, &lrow, &K);
+    const double nlog = -mc_log_k(h, &K);
     const double dq = cell.sig_i > MC_EPS ? nlog / cell.sig_i : MC_MAXREAL;
     collided = dq < di_edge;
     if (collided) di = dq;

@@ -123,7 +123,6 @@ typedef struct {
   double l8, l7, l6, l5, l4, l3; /* -1/8, 1/7, -1/6, 1/5, -1/4, 1/3 */
   double e5, e4, e3;             /* 1/5!, 1/4!, 1/3! */
   double ziv;                    /* MC_ZIV_E */
-  double a6, a5, a3;             /* mc_log_approx: -1/6, 1/5, 1/3 */
 } mc_coef;
 
 MC_FN mc_coef mc_coef_default(void) {
@@ -133,7 +132,6 @@ MC_FN mc_coef mc_coef_default(void) {
   K.e5 = mc_inv_fact[5][0];
   K.e4 = mc_inv_fact[4][0]; K.e3 = mc_inv_fact[3][0];
   K.ziv = MC_ZIV_E;
-  K.a6 = -1.0 / 6.0; K.a5 = 0.2; K.a3 = 1.0 / 3.0;
   return K;
 }
 
@@ -233,30 +231,6 @@ MC_FN mc_log_row mc_log_lookup(uint64_t ix, int eadj) {
   R.head = mc_log_tab[R.idx][1];
   R.mid = mc_log_tab[R.idx][2];
   return R;
-}
-
-/* log x from its table row in plain double, relative error below 2^-48
-   (not correctly rounded; for decisions that allow a margin).  With
-   L = log x, s = e*ln2_hi + head (exact) and lp = log1p(r):
-     lp = r + r^2 (-1/2 + r/3 - r^2/4 + r^3/5 - r^4/6): truncation below
-       |r|^6/7 of lp, 2^-54.8 for |r| < 2^-8.67, and five roundings below
-       2^-51.4 of lp;
-     |lp| <= 4 |L|: gen_math_tables.py asserts |log m| >= |head|/3 over every
-       row with c != 1 (rows with c = 1 have head = 0, and for e != 0,
-       |L| > 0.34 dominates);
-     the two additions round by at most 2^-53 (|L| + |t + lp|) <= 5 2^-53 |L|;
-     e*ln2_lo and the table's third part are below 2^-100 absolute.
-   In all: below 4 (2^-51.4 + 2^-54.8) + 2^-50.7 < 2^-48.6 of |L|. */
-MC_FN double mc_log_approx(const mc_log_row *R, const mc_coef *K) {
-  const double r = mc_fma(R->m, R->c, -1.0); /* exact */
-  double q = mc_fma(r, K->a6, K->a5);
-  q = mc_fma(r, q, -0.25);
-  q = mc_fma(r, q, K->a3);
-  q = mc_fma(r, q, -0.5);
-  const double lp = mc_fma(r * r, q, r);
-  const double ed = (double)R->e;
-  const double s = ed * MC_LN2_HI + R->head; /* exact */
-  return s + (mc_fma(ed, MC_LN2_MID, R->mid) + lp);
 }
 
 MC_FN double mc_log_eval(const mc_log_row *R, const mc_coef *K) {

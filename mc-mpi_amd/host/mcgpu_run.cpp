@@ -49,9 +49,13 @@ int main(int argc, char **argv) {
     printf("%lf\n", run.elapsed);
     run.dump(out);
     long long disabled = 0;
-    for (const mcgpu::Layer &L : run.layers) disabled += L.nb_disabled;
-    fprintf(stderr, "events %llu, disabled %lld, cycles %lld, ranks %d\n", run.events, disabled,
-            run.nb_cycles, ranks);
+    double kernel_ms = 0;
+    for (const mcgpu::Layer &L : run.layers) {
+      disabled += L.nb_disabled;
+      kernel_ms += L.total_stats.kernel_ms;
+    }
+    fprintf(stderr, "events %llu, disabled %lld, cycles %lld, ranks %d, kernel_ms %.3f\n",
+            run.events, disabled, run.nb_cycles, ranks, kernel_ms);
   } catch (const std::exception &e) {
     fprintf(stderr, "mcgpu_run: %s\n", e.what());
     return 1;

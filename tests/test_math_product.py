@@ -127,41 +127,86 @@ def test_math_tables_regenerate_identically():
         assert out == f.read()
 
 
-APPROX_SHIM = r"""
+CROSS_SHIM = r"""
+#include <math.h>
 #include <stdint.h>
 #include "mc_math.h"
-void pm_log_approx(const double *x, long n, double *y) {
-  const mc_coef K = mc_coef_default();
-  for (long i = 0; i < n; ++i) { const mc_log_row R = mc_log_start(x[i]); y[i] = mc_log_approx(&R, &K); }
+/* mode 0: the transport step's certificate, 1 - h > sig_i * di_edge * (1 + 2^-40);
+   mode 1: a planted wrong-direction bound, (1 - h) / h in place of 1 - h
+   (an upper bound of -log h, not a lower one). */
+void pm_cross(const double *h, const double *sig_i, const double *di_edge, long n, int mode,
+              int8_t *sure, int8_t *collided) {
+  for (long i = 0; i < n; ++i) {
+    const double reach = sig_i[i] * di_edge[i];
+    const double lower = mode == 0 ? 1.0 - h[i] : (1.0 - h[i]) / h[i];
+    sure[i] = !(sig_i[i] > 1e-4) || lower > fma(reach, 0x1p-40, reach);
+    const double dq = sig_i[i] > 1e-4 ? -mc_log(h[i]) / sig_i[i] : 1.7976931348623157e308;
+    collided[i] = dq < di_edge[i];
+  }
 }
 """
 
 
-def test_log_approx_error_bound(tmp_path, oracle):
-    """mc_log_approx decides crossings with a 2^-40 margin; it must stay
-    within its stated 2^-48 of the correctly rounded log over every table
-    row (dense near x = 1, where |log x| is smallest) and every exponent."""
-    src = tmp_path / "pa.c"
-    src.write_text(APPROX_SHIM)
-    so = tmp_path / "libpa.so"
+def test_crossing_certificate(tmp_path):
+    """The transport step skips the log when 1 - h > sig_i * di_edge *
+    (1 + 2^-40): since -log h >= 1 - h, the particle then crosses.  The
+    shortcut must never call a collision a crossing, checked against the
+    reference's comparison (correctly rounded -log(h) / sig_i < di_edge) on
+    random draws, on h next to 1 with di_edge placed at the boundary of the
+    test (ratios 1 +- 2^-k), and on the special inputs (h = 0 and 1, sig_i
+    at and below EPS, di_edge of 0, -0, negative and MAXREAL).  A planted
+    wrong-direction bound must fail the same check."""
+    src = tmp_path / "pc.c"
+    src.write_text(CROSS_SHIM)
+    so = tmp_path / "libpc.so"
     subprocess.run(["gcc", "-O2", "-std=gnu11", "-fPIC", "-shared", "-ffp-contract=off",
                     "-I", os.path.join(ROOT, "mc-mpi_amd", "csrc"), "-o", str(so), str(src), "-lm"],
                    check=True)
     L = C.CDLL(str(so))
-    L.pm_log_approx.argtypes = [C.c_void_p, C.c_long, C.c_void_p]
-    rng = np.random.default_rng(9)
-    s = rng.integers(1, 2 ** 63, 300_000, dtype=np.uint64)
-    h = s.astype(np.float64) * 2.0 ** -63                      # rnd_real outputs
-    near = 1.0 + (rng.random(300_000) - 0.5) * 2.0 ** -4        # rows next to 1
-    tiny = 1.0 - rng.integers(1, 2 ** 20, 50_000).astype(np.float64) * 2.0 ** -53
-    x = np.concatenate([h, near, np.ldexp(near, rng.integers(-60, 60, 300_000)), tiny])
-    x = x[x != 1.0]
-    y = np.empty_like(x)
-    L.pm_log_approx(x.ctypes.data, len(x), y.ctypes.data)
-    ref = oracle.math_eval("log", x)
-    rel = np.abs(y - ref) / np.abs(ref)
-    print("max rel error 2^%.2f" % np.log2(rel.max()))
-    assert rel.max() < 2.0 ** -48, np.log2(rel.max())
+    L.pm_cross.argtypes = [C.c_void_p] * 3 + [C.c_long, C.c_int, C.c_void_p, C.c_void_p]
+    rng = np.random.default_rng(11)
+    n = 300_000
+    s = rng.integers(0, 2 ** 63, n, dtype=np.uint64)
+    h_rand = s.astype(np.float64) * 2.0 ** -63
+    h_near = 1.0 - rng.integers(1, 2 ** 24, n).astype(np.float64) * 2.0 ** -53
+    h_mid = rng.random(n)
+    hs, sigs, dis = [], [], []
+    for h in (h_rand, h_near, h_mid):
+        sig = np.exp(rng.normal(0.0, 1.5, n))
+        k = rng.integers(8, 62, n).astype(np.float64)
+        ratio = 1.0 + rng.choice([-1.0, 1.0], n) * np.ldexp(rng.random(n), -k.astype(int))
+        ratio = np.where(rng.random(n) < 0.3, rng.random(n) * 3.0, ratio)
+        hs.append(h)
+        sigs.append(sig)
+        dis.append(ratio * (1.0 - h) / sig)
+        # di_edge at -log(h) / sig_i: inside the band where a wrong-direction
+        # bound claims a crossing
+        hs.append(h)
+        sigs.append(sig)
+        dis.append(-np.log(np.maximum(h, 1e-300)) / sig * (1.0 + 2.0 ** -20))
+    special_h = np.array([0.0, 1.0, 2.0 ** -63, 1.0 - 2.0 ** -53, 0.5])
+    special_sig = np.array([0.0, 1e-4, 5e-5, np.nextafter(1e-4, 1.0), 1.0, 1e300])
+    special_di = np.array([0.0, -0.0, -1e-3, 1.7976931348623157e308, 1e-300, 1.0])
+    g = np.array(np.meshgrid(special_h, special_sig, special_di)).reshape(3, -1)
+    hs.append(g[0])
+    sigs.append(g[1])
+    dis.append(g[2])
+    h = np.ascontiguousarray(np.concatenate(hs))
+    sig = np.ascontiguousarray(np.concatenate(sigs))
+    di = np.ascontiguousarray(np.concatenate(dis))
+    sure = np.empty(len(h), np.int8)
+    coll = np.empty(len(h), np.int8)
+
+    def run(mode):
+        L.pm_cross(h.ctypes.data, sig.ctypes.data, di.ctypes.data, len(h), mode,
+                   sure.ctypes.data, coll.ctypes.data)
+        return sure.astype(bool), coll.astype(bool)
+
+    sure0, coll0 = run(0)
+    assert not np.any(sure0 & coll0)
+    assert sure0.mean() > 0.1 and coll0.mean() > 0.1      # both outcomes exercised
+    sure1, coll1 = run(1)
+    assert np.count_nonzero(sure1 & coll1) > 1000           # the planted bound is caught
 
 
 DIV_SHIM = r"""
