@@ -283,6 +283,74 @@ int mcgpu_create_particles(mcgpu_context *ctx, uint64_t seed, int64_t first, int
                            double x_ini, double wmc, double dx, const mcgpu_bank *bank,
                            void *stream);
 
+/* ------------------------------------------------------------- cycle loop
+ * The synchronous loop of the reference's workers (WorkerSync::spin,
+ * src/worker_sync.cpp:24-135, all ranks of `mpirun -n world_size ./main
+ * config.yaml sync`) with every rank's layer resident on this GPU.  Per
+ * cycle, each rank r does Layer::simulate(nb_particles_per_cycle)
+ * (src/layer.cpp:239-361): lazy creation from its source when its bank holds
+ * fewer than nb_particles_per_cycle particles (src/layer.cpp:89-121), then
+ * transport of the last min(size, nb_particles_per_cycle) particles, whose
+ * escapes form particles_left / particles_right in the reference's order
+ * (iteration i transports particles[size-1-i]); escapes through a global
+ * border (left_border / right_border) and kills count as disabled.  Then the
+ * two exchange phases (odd ranks with their right neighbour, then even
+ * ranks, src/worker_sync.cpp:55-109) append each escape buffer to the
+ * neighbour's bank in sender order.  The loop ends when the disabled counts
+ * add up to nb_particles (src/worker_sync.cpp:111-119).
+ * All ranks' transports of one cycle share one launch (the batched
+ * kernel's work queue), and banks, escape buffers and the exchange stay in
+ * device memory: per cycle the host reads back one small summary.
+ * Per history and per cycle the arithmetic is that of mcgpu_simulate, so
+ * the tallies (one rounding into weights_absorbed per rank and cycle, as
+ * Layer::simulate adds its call's tally) and the counts are bit-identical to
+ * calling mcgpu_simulate per rank and cycle and exchanging on the host.
+ * No counterpart exists in the reference's API (its ranks are processes).
+ */
+typedef struct mcgpu_cycle_rank {
+  /* in: the rank's Layer */
+  int32_t min_index;   /* global cell range [min_index, max_index) */
+  int32_t max_index;
+  double dx;
+  double particle_min_weight;
+  int32_t left_border; /* Layer::left_border / right_border */
+  int32_t right_border;
+  int64_t cell_offset; /* the rank's cells in the concatenated sigs / ar / weights */
+  const mcgpu_particle *particles; /* host: the initial bank, particles[0..n_particles) */
+  int64_t n_particles;
+  /* in / out: the lazy source (Layer::create_particles(x_ini, wmc, n, seed)) */
+  int64_t nb_particles_create;
+  uint64_t seed;       /* rnd_seed chain state */
+  double x_ini, wmc;
+  /* in / out */
+  int64_t nb_disabled;
+  /* out: totals over the run */
+  int64_t n_left, n_right, n_killed; /* escapes (either border or exchanged) and kills */
+  int64_t n_remaining;               /* particles left in the bank */
+} mcgpu_cycle_rank;
+
+typedef struct mcgpu_cycle_stats {
+  int64_t cycles;
+  uint64_t events;
+  uint64_t collisions;
+  double kernel_ms;    /* the transport launches */
+  double wall_ms;      /* the whole loop */
+} mcgpu_cycle_stats;
+
+/* Runs cycles until the ranks' nb_disabled add up to nb_particles.
+   sigs, absorption_rates, weights_absorbed: host arrays of total_cells (the
+   ranks' cells at their cell_offset); weights_absorbed is added to.
+   nb_particles_per_cycle must be in [1, 1000000] (MAX_PARTICLES_VECTOR).
+   opt: block_size (default 256), particles_per_lane (default 1),
+   events_per_refill, grid_blocks and max_events_per_history of the
+   transport launch.  Returns MCGPU_ERR_ARG (and leaves the ranks' outputs
+   as of the last cycle) if max_cycles > 0 cycles pass without termination. */
+int mcgpu_sync_cycle(mcgpu_context *ctx, int32_t world_size, mcgpu_cycle_rank *ranks,
+                     const double *sigs, const double *absorption_rates,
+                     double *weights_absorbed, int64_t total_cells,
+                     int32_t nb_particles_per_cycle, int64_t nb_particles, int64_t max_cycles,
+                     const mcgpu_options *opt, mcgpu_cycle_stats *stats);
+
 /* Evaluate the device log (which = 0) or exp (which = 1) used by the
    transport kernels on n inputs (device pointers). */
 int mcgpu_math_eval(mcgpu_context *ctx, int which, int64_t n, const double *x, double *out,

@@ -25,24 +25,9 @@
 
 namespace {
 
-struct slab_k {
-  int32_t index_start;
-  int32_t m;
-  double dx;
-  double min_weight;
-  int64_t cell_off;
-};
-
-struct tile_k {
-  int32_t slab;
-  int32_t pad_;
-  int64_t begin; /* absolute particle indices [begin, end) */
-  int64_t end;
-};
-
 struct batch_args {
-  const slab_k *slabs;
-  const tile_k *tiles;
+  const mc_slab_k *slabs;
+  const mc_tile_k *tiles;
   int32_t n_tiles;
   int32_t lds_cells_max_m; /* slabs with m <= this stage cross sections in LDS */
   int32_t lds_tally_max_m; /* slabs with m <= this keep their tally in LDS; larger
@@ -67,8 +52,8 @@ struct batch_lane {
    tally and cell pointers are LDS or global according to the template
    arguments, so every access compiles to the matching instruction. */
 template <bool LDS_TALLY, bool LDS_CELLS>
-__device__ __forceinline__ void batch_tile(const batch_args &a, const tile_k &tile,
-                                           const slab_k &sl, const mc_layer_k &L,
+__device__ __forceinline__ void batch_tile(const batch_args &a, const mc_tile_k &tile,
+                                           const mc_slab_k &sl, const mc_layer_k &L,
                                            unsigned long long *t_lo, unsigned long long *t_hi,
                                            const mc_cell *cells, unsigned long long *s_next,
                                            unsigned long long *s_tot, uint32_t max_ev,
@@ -166,8 +151,8 @@ __global__ __launch_bounds__(1024) void k_batched(batch_args a, int refill_k) {
     __syncthreads();
     const int t = s_tile;
     if (t >= a.n_tiles) break; /* block-uniform */
-    const tile_k tile = a.tiles[t];
-    const slab_k sl = a.slabs[tile.slab];
+    const mc_tile_k tile = a.tiles[t];
+    const mc_slab_k sl = a.slabs[tile.slab];
     const int m = sl.m;
     mc_layer_k L;
     L.index_start = sl.index_start;
@@ -204,7 +189,8 @@ __global__ __launch_bounds__(1024) void k_batched(batch_args a, int refill_k) {
         mc_fx v;
         v.lo = t_lds[c];
         v.hi = t_lds[m + c];
-        mc_fx_atomic_add(&a.fx[sl.cell_off + c], &a.fx[a.total_cells + sl.cell_off + c], v);
+        if (v.lo | v.hi) /* cells the tile never reached add nothing */
+          mc_fx_atomic_add(&a.fx[sl.cell_off + c], &a.fx[a.total_cells + sl.cell_off + c], v);
       }
     }
     __syncthreads();
@@ -256,6 +242,60 @@ __global__ void k_batch_finalize(int64_t total, const unsigned long long *fx,
 
 } // namespace
 
+int mc_batched_plan_shape(mcgpu_context *ctx, int max_m, int block, int grid, int refill,
+                          mc_batched_shape *out) {
+  if (block > 1024 || block < 64 || block % 64)
+    return mc_fail(ctx, MCGPU_ERR_ARG, "block_size must be a multiple of 64 and <= 1024");
+  mc_batched_shape sh;
+  sh.block = block;
+  sh.refill = refill > 0 ? refill : 8;
+  /* LDS: the largest slab's tally plus its cross sections, as far as they fit
+     (128 B are left for the kernel's static LDS); slabs whose tally does not
+     fit add into the global tally directly */
+  const size_t budget = ctx->lds_per_block - 128;
+  sh.lds = std::min(budget, (size_t)32 * (size_t)std::max(max_m, 1));
+  sh.lds_cells_max_m = (int)(sh.lds / 32);
+  sh.lds_tally_max_m = (int)(sh.lds / 16);
+  if (sh.lds > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_batched),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.lds);
+  sh.grid = grid;
+  if (sh.grid <= 0) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batched, block, sh.lds) !=
+            hipSuccess ||
+        per_cu < 1)
+      per_cu = 1;
+    sh.grid = per_cu * ctx->num_cus;
+  }
+  *out = sh;
+  return MCGPU_OK;
+}
+
+int mc_launch_batched(mcgpu_context *ctx, const mc_batched_shape &sh, const mc_slab_k *slabs,
+                      const mc_tile_k *tiles, int32_t n_tiles, const mcgpu_bank &bank,
+                      const mc_cell *cells, unsigned long long *fx, int64_t total_cells,
+                      uint32_t *flags, int8_t *result8, mc_counters *ctr, uint64_t max_events,
+                      hipStream_t s) {
+  batch_args a;
+  a.slabs = slabs;
+  a.tiles = tiles;
+  a.n_tiles = n_tiles;
+  a.lds_cells_max_m = sh.lds_cells_max_m;
+  a.lds_tally_max_m = sh.lds_tally_max_m;
+  a.soa = bank;
+  a.cells = cells;
+  a.fx = fx;
+  a.total_cells = total_cells;
+  a.flags = flags;
+  a.result8 = result8;
+  a.ctr = ctr;
+  a.max_events = max_events;
+  hipLaunchKernelGGL(k_batched, dim3(sh.grid), dim3(sh.block), sh.lds, s, a, sh.refill);
+  MC_HIP(ctx, hipGetLastError());
+  return MCGPU_OK;
+}
+
 struct mcgpu_batch {
   mcgpu_context *ctx = nullptr;
   std::vector<mcgpu_slab> host_slabs;
@@ -267,8 +307,8 @@ struct mcgpu_batch {
   mcgpu_bank bank{};
   int8_t *result = nullptr;
   /* plan-owned device memory */
-  slab_k *d_slabs = nullptr;
-  tile_k *d_tiles = nullptr;
+  mc_slab_k *d_slabs = nullptr;
+  mc_tile_k *d_tiles = nullptr;
   int32_t n_tiles = 0;
   size_t tiles_cap = 0;
   mc_cell *d_cells = nullptr;
@@ -293,13 +333,13 @@ int batch_tiles(mcgpu_batch *b) {
   mcgpu_context *ctx = b->ctx;
   const int ppl = b->opt.particles_per_lane > 0 ? b->opt.particles_per_lane : 16;
   const int64_t T = (int64_t)ppl * b->block;
-  std::vector<tile_k> tiles;
+  std::vector<mc_tile_k> tiles;
   std::vector<double> work;
   for (size_t s = 0; s < b->host_slabs.size(); ++s) {
     const mcgpu_slab &h = b->host_slabs[s];
     const int m = h.max_index - h.min_index;
     for (int64_t p0 = 0; p0 < h.n; p0 += T) {
-      tile_k t;
+      mc_tile_k t;
       t.slab = (int32_t)s;
       t.pad_ = 0;
       t.begin = h.particle_offset + p0;
@@ -311,17 +351,17 @@ int batch_tiles(mcgpu_batch *b) {
   std::vector<size_t> order(tiles.size());
   for (size_t i = 0; i < order.size(); ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return work[x] > work[y]; });
-  std::vector<tile_k> sorted(tiles.size());
+  std::vector<mc_tile_k> sorted(tiles.size());
   for (size_t i = 0; i < order.size(); ++i) sorted[i] = tiles[order[i]];
   b->n_tiles = (int32_t)sorted.size();
   if (sorted.size() > b->tiles_cap) {
     if (b->d_tiles) MC_HIP(ctx, hipFree(b->d_tiles));
     b->d_tiles = nullptr;
-    MC_HIP(ctx, hipMalloc(&b->d_tiles, sizeof(tile_k) * (sorted.size() + 1)));
+    MC_HIP(ctx, hipMalloc(&b->d_tiles, sizeof(mc_tile_k) * (sorted.size() + 1)));
     b->tiles_cap = sorted.size();
   }
   if (!sorted.empty())
-    MC_HIP(ctx, hipMemcpy(b->d_tiles, sorted.data(), sizeof(tile_k) * sorted.size(),
+    MC_HIP(ctx, hipMemcpy(b->d_tiles, sorted.data(), sizeof(mc_tile_k) * sorted.size(),
                           hipMemcpyHostToDevice));
   return MCGPU_OK;
 }
@@ -334,25 +374,13 @@ int batch_capture(mcgpu_batch *b) {
   b->refill = b->opt.events_per_refill > 0 ? b->opt.events_per_refill : 8;
   int rc = batch_tiles(b);
   if (rc) return rc;
-  /* LDS: the largest slab's tally plus its cross sections, as far as they fit
-     (128 B are left for the kernel's static LDS); slabs whose tally does not
-     fit add into the global tally directly */
-  const size_t budget = ctx->lds_per_block - 128;
-  b->lds = std::min(budget, (size_t)32 * b->max_m);
-  b->lds_cells_max_m = (int)(b->lds / 32);
-  b->lds_tally_max_m = (int)(b->lds / 16);
-  if (b->lds > 65536)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_batched),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)b->lds);
-  b->grid = b->opt.grid_blocks;
-  if (b->grid <= 0) {
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batched, b->block, b->lds) !=
-            hipSuccess ||
-        per_cu < 1)
-      per_cu = 1;
-    b->grid = per_cu * ctx->num_cus;
-  }
+  mc_batched_shape sh;
+  if ((rc = mc_batched_plan_shape(ctx, b->max_m, b->block, b->opt.grid_blocks, b->refill, &sh)))
+    return rc;
+  b->lds = sh.lds;
+  b->lds_cells_max_m = sh.lds_cells_max_m;
+  b->lds_tally_max_m = sh.lds_tally_max_m;
+  b->grid = sh.grid;
   if (b->exec) {
     (void)hipGraphExecDestroy(b->exec);
     b->exec = nullptr;
@@ -379,7 +407,7 @@ int batch_capture(mcgpu_batch *b) {
   MC_HIP(ctx, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   hipLaunchKernelGGL(k_batch_setup, dim3((unsigned)((tc + 255) / 256)), dim3(256), 0, b->stream,
                      tc, b->sigs, b->ar, b->d_cells, b->d_fx, b->d_flags, b->d_ctr);
-  hipLaunchKernelGGL(k_batched, dim3(b->grid), dim3(b->block), b->lds, b->stream, a, b->refill);
+  hipLaunchKernelGGL(k_batched, dim3(sh.grid), dim3(sh.block), sh.lds, b->stream, a, sh.refill);
   hipLaunchKernelGGL(k_batch_finalize, dim3((unsigned)((tc + 255) / 256)), dim3(256), 0,
                      b->stream, tc, (const unsigned long long *)b->d_fx,
                      (const uint32_t *)b->d_flags, b->weights);
@@ -459,7 +487,7 @@ int mcgpu_batch_create(mcgpu_context *ctx, int32_t n_slabs, const mcgpu_slab *sl
   b->total_cells = total_cells;
   b->bank = *bank;
   b->result = result;
-  std::vector<slab_k> ks(n_slabs);
+  std::vector<mc_slab_k> ks(n_slabs);
   int rc = MCGPU_OK;
   for (int s = 0; s < n_slabs && !rc; ++s) {
     const mcgpu_slab &h = slabs[s];
@@ -483,8 +511,8 @@ int mcgpu_batch_create(mcgpu_context *ctx, int32_t n_slabs, const mcgpu_slab *sl
     hipError_t e_ = (call);                                                   \
     if (e_ != hipSuccess) rc = mc_fail(ctx, MCGPU_ERR_HIP, #call, e_);        \
   }
-  B_HIP(hipMalloc(&b->d_slabs, sizeof(slab_k) * n_slabs));
-  B_HIP(hipMemcpy(b->d_slabs, ks.data(), sizeof(slab_k) * n_slabs, hipMemcpyHostToDevice));
+  B_HIP(hipMalloc(&b->d_slabs, sizeof(mc_slab_k) * n_slabs));
+  B_HIP(hipMemcpy(b->d_slabs, ks.data(), sizeof(mc_slab_k) * n_slabs, hipMemcpyHostToDevice));
   B_HIP(hipMalloc(&b->d_cells, sizeof(mc_cell) * total_cells));
   B_HIP(hipMalloc(&b->d_fx, 16 * (size_t)total_cells));
   B_HIP(hipMalloc(&b->d_flags, 4 * (size_t)total_cells));

@@ -88,30 +88,11 @@ __global__ void k_rng_real_step(int64_t n, uint64_t *seeds, double *reals) {
   }
 }
 
-/* The k-th output of rnd_seed from `seed` (src/random.cpp:24-29), k >= 1,
-   via the affine map f(s) = G s + C raised to the power k by squaring; the
-   map is taken mod 2^64 and reduced mod 2^63 at the end, which is exact
-   because 2^63 divides 2^64. */
-__device__ __forceinline__ uint64_t seed_chain_at(uint64_t seed, uint64_t k) {
-  uint64_t A = MC_RNGS_G, B = MC_RNGS_C; /* f^(2^j) */
-  uint64_t RA = 1, RB = 0;               /* accumulated power */
-  while (k) {
-    if (k & 1) {
-      RA = A * RA;
-      RB = A * RB + B;
-    }
-    B = A * B + B;
-    A = A * A;
-    k >>= 1;
-  }
-  return (RA * seed + RB) & MC_MASK63;
-}
-
 /* out[i] = (i+1)-th output of rnd_seed from `seed`. */
 __global__ void k_seed_chain(uint64_t seed, int64_t n, uint64_t *out) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  out[i] = seed_chain_at(seed, (uint64_t)i + 1);
+  out[i] = mc_seed_chain_at(seed, (uint64_t)i + 1);
 }
 
 /* Layer::create_particles(int) (src/layer.cpp:89-121) for particles
@@ -121,7 +102,7 @@ __global__ void k_create_particles(uint64_t seed, int64_t first, int64_t n, doub
                                    double wmc, int32_t index, mcgpu_bank b) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  uint64_t s = seed_chain_at(seed, (uint64_t)(first + i) + 1);
+  uint64_t s = mc_seed_chain_at(seed, (uint64_t)(first + i) + 1);
   const double mu = 2 * mc_rnd_real(s) - 1;
   b.seed[i] = s;
   b.x[i] = x_ini;
@@ -212,6 +193,15 @@ int check_layer(mcgpu_context *ctx, int32_t min_index, int32_t max_index, double
 }
 
 } // namespace
+
+int mc_launch_create(mcgpu_context *ctx, uint64_t seed, int64_t first, int64_t n, double x_ini,
+                     double wmc, int32_t index, const mcgpu_bank &bank, hipStream_t s) {
+  if (n <= 0) return MCGPU_OK;
+  hipLaunchKernelGGL(k_create_particles, dim3(grid_for(n, 256)), dim3(256), 0, s, seed, first, n,
+                     x_ini, wmc, index, bank);
+  MC_HIP(ctx, hipGetLastError());
+  return MCGPU_OK;
+}
 
 /* =========================================================== C ABI */
 extern "C" {

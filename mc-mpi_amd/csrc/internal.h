@@ -119,4 +119,47 @@ int mc_launch_history(mcgpu_context *ctx, const mc_transport_args &a, const mcgp
 int mc_launch_event(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_options &o,
                     hipStream_t s);
 
+/* Particles first .. first+n-1 of a lazy source (Layer::create_particles,
+   src/layer.cpp:89-121) into bank[0..n), on stream s (capi.hip). */
+int mc_launch_create(mcgpu_context *ctx, uint64_t seed, int64_t first, int64_t n, double x_ini,
+                     double wmc, int32_t index, const mcgpu_bank &bank, hipStream_t s);
+
+/* The batched transport (batched.hip): a persistent launch over a work
+   queue of tiles, each a particle range [begin, end) of the bank belonging
+   to one slab (a layer: global cell range, dx, min weight, offset of its
+   cells in the concatenated cell / tally arrays). */
+struct mc_slab_k {
+  int32_t index_start;
+  int32_t m;
+  double dx;
+  double min_weight;
+  int64_t cell_off;
+};
+
+struct mc_tile_k {
+  int32_t slab;
+  int32_t pad_;
+  int64_t begin; /* absolute particle indices [begin, end) */
+  int64_t end;
+};
+
+struct mc_batched_shape {
+  int block, grid, refill;
+  size_t lds;          /* dynamic LDS bytes per block */
+  int lds_cells_max_m; /* slabs with m <= this stage their cells in LDS */
+  int lds_tally_max_m; /* slabs with m <= this keep their tally in LDS */
+};
+
+/* Launch shape for slabs of up to max_m cells; grid <= 0 takes resident
+   blocks per CU x CUs. */
+int mc_batched_plan_shape(mcgpu_context *ctx, int max_m, int block, int grid, int refill,
+                          mc_batched_shape *out);
+/* One launch over n_tiles tiles; fx / flags / ctr must be zeroed first
+   (ctr->next is the tile queue head). */
+int mc_launch_batched(mcgpu_context *ctx, const mc_batched_shape &sh, const mc_slab_k *slabs,
+                      const mc_tile_k *tiles, int32_t n_tiles, const mcgpu_bank &bank,
+                      const mc_cell *cells, unsigned long long *fx, int64_t total_cells,
+                      uint32_t *flags, int8_t *result8, mc_counters *ctr, uint64_t max_events,
+                      hipStream_t s);
+
 #endif

@@ -40,6 +40,25 @@ __device__ __forceinline__ uint64_t mc_rnd_seed(uint64_t &seed) {
   return seed;
 }
 
+/* The k-th output of rnd_seed from `seed` (src/random.cpp:24-29), k >= 1,
+   via the affine map f(s) = G s + C raised to the power k by squaring; the
+   map is taken mod 2^64 and reduced mod 2^63 at the end, which is exact
+   because 2^63 divides 2^64. */
+__host__ __device__ __forceinline__ uint64_t mc_seed_chain_at(uint64_t seed, uint64_t k) {
+  uint64_t A = MC_RNGS_G, B = MC_RNGS_C; /* f^(2^j) */
+  uint64_t RA = 1, RB = 0;               /* accumulated power */
+  while (k) {
+    if (k & 1) {
+      RA = A * RA;
+      RB = A * RB + B;
+    }
+    B = A * B + B;
+    A = A * A;
+    k >>= 1;
+  }
+  return (RA * seed + RB) & MC_MASK63;
+}
+
 /* ------------------------------------------------------ cross sections */
 /* Per cell, the two products particle_step forms (src/layer.cpp:131-133):
    sig_a = sigs*ar and sig_i = sigs*(1.0 - ar), computed once with the same

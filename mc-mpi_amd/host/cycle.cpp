@@ -7,6 +7,7 @@
 #include <sys/stat.h>
 #include <unistd.h>
 
+#include <algorithm>
 #include <chrono>
 
 namespace mcgpu {
@@ -43,6 +44,71 @@ SyncCycle::SyncCycle(const MCMPIOptions &opt) : options(opt) {
 }
 
 long long SyncCycle::spin(long long max_cycles) {
+  if (device_resident && options.nb_particles_per_cycle <= MAX_PARTICLES_VECTOR)
+    return spin_device(max_cycles);
+  return spin_host(max_cycles);
+}
+
+long long SyncCycle::spin_device(long long max_cycles) {
+  const int R = options.world_size;
+  std::vector<mcgpu_cycle_rank> rk(R);
+  std::vector<int64_t> off(R + 1, 0);
+  for (int r = 0; r < R; ++r) off[r + 1] = off[r] + layers[r].get_m();
+  const int64_t total = off[R];
+  std::vector<real_t> sigs(total), ar(total), w(total);
+  for (int r = 0; r < R; ++r) {
+    Layer &L = layers[r];
+    std::copy(L.sigs.begin(), L.sigs.end(), sigs.begin() + off[r]);
+    std::copy(L.absorption_rates.begin(), L.absorption_rates.end(), ar.begin() + off[r]);
+    std::copy(L.weights_absorbed.begin(), L.weights_absorbed.end(), w.begin() + off[r]);
+    if (!L.particles_left.empty() || !L.particles_right.empty())
+      throw Error(MCGPU_ERR_ARG, "SyncCycle: escape buffers must be empty before spin()");
+    mcgpu_cycle_rank &k = rk[r];
+    k = mcgpu_cycle_rank{};
+    k.min_index = L.get_index_start();
+    k.max_index = L.get_index_start() + L.get_m();
+    k.dx = L.dx;
+    k.particle_min_weight = L.particle_min_weight;
+    k.left_border = L.left_border;
+    k.right_border = L.right_border;
+    k.cell_offset = off[r];
+    k.particles = L.particles.data();
+    k.n_particles = (int64_t)L.particles.size();
+    k.nb_particles_create = L.nb_particles_create;
+    k.seed = L.seed;
+    k.x_ini = L.x_ini;
+    k.wmc = L.wmc;
+    k.nb_disabled = L.nb_disabled;
+  }
+  mcgpu_cycle_stats st{};
+  mcgpu_context *ctx = default_context();
+  const int rc = mcgpu_sync_cycle(ctx, R, rk.data(), sigs.data(), ar.data(), w.data(), total,
+                                  options.nb_particles_per_cycle, options.nb_particles, max_cycles,
+                                  &device_options, &st);
+  for (int r = 0; r < R; ++r) {
+    Layer &L = layers[r];
+    const mcgpu_cycle_rank &k = rk[r];
+    std::copy(w.begin() + off[r], w.begin() + off[r + 1], L.weights_absorbed.begin());
+    L.nb_disabled = (int)k.nb_disabled;
+    L.nb_particles_create = (int)k.nb_particles_create;
+    L.seed = k.seed;
+    if (k.n_remaining == 0) L.particles.clear();
+    L.total_stats.n_left += k.n_left;
+    L.total_stats.n_right += k.n_right;
+    L.total_stats.n_killed += k.n_killed;
+    times[r].comp += st.kernel_ms * 1e-3;
+    times[r].send += (st.wall_ms - st.kernel_ms) * 1e-3;
+  }
+  events += st.events;
+  nb_cycles += st.cycles;
+  elapsed = st.wall_ms * 1e-3;
+  if (rc != MCGPU_OK)
+    throw Error(rc, std::string("mcgpu_sync_cycle: ") + mcgpu_status_string(rc) + ": " +
+                        mcgpu_context_error(ctx));
+  return st.cycles;
+}
+
+long long SyncCycle::spin_host(long long max_cycles) {
   const int R = options.world_size;
   const double t_start = now();
   long long cycles = 0;

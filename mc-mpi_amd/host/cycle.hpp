@@ -36,8 +36,23 @@ public:
   explicit SyncCycle(const MCMPIOptions &options);
 
   /* WorkerSync::spin; returns the number of cycles.  Throws mcgpu::Error if
-     max_cycles (> 0) pass without every particle being disabled. */
+     max_cycles (> 0) pass without every particle being disabled.
+     With device_resident (the default) the whole loop runs on the GPU
+     (mcgpu_sync_cycle: banks, escape buffers and the exchange in device
+     memory, all ranks' transports of a cycle in one launch); otherwise each
+     rank's Layer::simulate is called in turn and the escape buffers move
+     between the Layer objects in host memory.  Both give the same cycles,
+     events, nb_disabled and weights_absorbed, bit for bit.  On return from
+     the device-resident loop every Layer holds its final state (tallies,
+     nb_disabled, source, empty banks); its per-call stats (last_stats /
+     total_stats) count escapes and kills per rank, and events only in
+     SyncCycle::events.  nb_particles_per_cycle above MAX_PARTICLES_VECTOR
+     takes the host loop (Layer::simulate then chunks each call). */
   long long spin(long long max_cycles = -1);
+
+  bool device_resident = true;
+  /* launch shape of the device-resident loop's transport (0 = defaults) */
+  mcgpu_options device_options{};
 
   /* Worker::gather_weights_absorbed (src/worker.cpp:183-216): the ranks'
      tallies concatenated in rank order; displs receives each rank's offset. */
@@ -56,6 +71,10 @@ public:
   };
   std::vector<RankTimes> times;
   double elapsed = 0; /* seconds of the last spin() */
+
+private:
+  long long spin_host(long long max_cycles);
+  long long spin_device(long long max_cycles);
 };
 
 } // namespace mcgpu

@@ -6,7 +6,11 @@
  * out/config.yaml, out/weights.csv, out/stats.csv and WA.out of rank 0.
  * Prints the elapsed seconds on stdout, like src/main.cpp:90-91.
  *
- *   mcgpu_run config.yaml [history|event] [device] [--ranks R] [--out DIR]
+ *   mcgpu_run config.yaml [history|event] [device] [--ranks R] [--out DIR] [--host-loop]
+ *
+ * By default the loop runs device-resident (mcgpu_sync_cycle); --host-loop
+ * calls each rank's Layer::simulate in turn and exchanges on the host (the
+ * kernel argument applies to that loop).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -18,15 +22,20 @@
 
 int main(int argc, char **argv) {
   if (argc < 2) {
-    fprintf(stderr, "usage: %s config.yaml [history|event] [device] [--ranks R] [--out DIR]\n",
+    fprintf(stderr,
+            "usage: %s config.yaml [history|event] [device] [--ranks R] [--out DIR] "
+            "[--host-loop]\n",
             argv[0]);
     return 2;
   }
   int ranks = 1, kernel = MCGPU_KERNEL_HISTORY, device = -1, positional = 0;
+  bool host_loop = false;
   std::string out = "out";
   for (int i = 2; i < argc; ++i) {
     if (!strcmp(argv[i], "--ranks") && i + 1 < argc) {
       ranks = atoi(argv[++i]);
+    } else if (!strcmp(argv[i], "--host-loop")) {
+      host_loop = true;
     } else if (!strcmp(argv[i], "--out") && i + 1 < argc) {
       out = argv[++i];
     } else if (positional == 0) {
@@ -45,6 +54,7 @@ int main(int argc, char **argv) {
     const mcgpu::MCMPIOptions opt = mcgpu::options_from_config(argv[1], ranks);
     mcgpu::SyncCycle run(opt);
     for (mcgpu::Layer &L : run.layers) L.options.kernel = kernel;
+    run.device_resident = !host_loop;
     run.spin();
     printf("%lf\n", run.elapsed);
     run.dump(out);
@@ -54,8 +64,9 @@ int main(int argc, char **argv) {
       disabled += L.nb_disabled;
       kernel_ms += L.total_stats.kernel_ms;
     }
-    fprintf(stderr, "events %llu, disabled %lld, cycles %lld, ranks %d, kernel_ms %.3f\n",
-            run.events, disabled, run.nb_cycles, ranks, kernel_ms);
+    if (!host_loop) kernel_ms = run.times.empty() ? 0.0 : run.times[0].comp * 1e3;
+    fprintf(stderr, "events %llu, disabled %lld, cycles %lld, ranks %d, kernel_ms %.3f, loop %s\n",
+            run.events, disabled, run.nb_cycles, ranks, kernel_ms, host_loop ? "host" : "device");
   } catch (const std::exception &e) {
     fprintf(stderr, "mcgpu_run: %s\n", e.what());
     return 1;

@@ -9,8 +9,8 @@ import ctypes as C
 import numpy as np
 
 from ._lib import (ERR_ARG, ERR_EVENT_CAP, ERR_HIP, ERR_NO_DEVICE, ERR_NOMEM, ERR_RANGE,
-                   KERNEL_EVENT, KERNEL_HISTORY, OK, PARTICLE, Bank, McgpuError, Options, Slab,
-                   Stats, lib, status_string)
+                   KERNEL_EVENT, KERNEL_HISTORY, OK, PARTICLE, Bank, CycleRank, CycleStats,
+                   McgpuError, Options, Slab, Stats, lib, status_string)
 
 __all__ = ["Context", "Batch", "PARTICLE", "McgpuError", "KERNEL_HISTORY", "KERNEL_EVENT", "lib",
            "bank_to_device", "bank_from_device", "make_options", "options_dict"]
@@ -225,6 +225,46 @@ class Context:
                                              _stream(stream))
         self.check(rc)
         return st.as_dict()
+
+    # ------------------------------------------------------------ cycle loop
+    def sync_cycle(self, ranks, sigs, absorption_rates, weights_absorbed, nb_particles_per_cycle,
+                   nb_particles, max_cycles=0, options=None):
+        """mcgpu_sync_cycle: the synchronous multi-rank loop, every rank on
+        this GPU.  `ranks`: one dict per rank with min_index, max_index, dx,
+        particle_min_weight, left_border, right_border, cell_offset,
+        particles (PARTICLE array, the initial bank), and optionally the lazy
+        source nb_particles_create, seed, x_ini, wmc and nb_disabled.
+        sigs / absorption_rates / weights_absorbed: host float64 arrays of
+        all ranks' cells; weights_absorbed is added to in place.  Returns
+        (per-rank output dicts, stats dict)."""
+        assert weights_absorbed.dtype == np.float64 and weights_absorbed.flags.c_contiguous
+        sigs = np.ascontiguousarray(sigs, np.float64)
+        ar = np.ascontiguousarray(absorption_rates, np.float64)
+        arr = (CycleRank * len(ranks))()
+        keep = []
+        for k, d in zip(arr, ranks):
+            p = np.ascontiguousarray(d.get("particles", np.zeros(0, PARTICLE)), PARTICLE)
+            keep.append(p)
+            k.min_index, k.max_index = d["min_index"], d["max_index"]
+            k.dx, k.particle_min_weight = d["dx"], d["particle_min_weight"]
+            k.left_border, k.right_border = int(d["left_border"]), int(d["right_border"])
+            k.cell_offset = d["cell_offset"]
+            k.particles, k.n_particles = (p.ctypes.data if len(p) else None), len(p)
+            k.nb_particles_create = d.get("nb_particles_create", 0)
+            k.seed = d.get("seed", 0)
+            k.x_ini, k.wmc = d.get("x_ini", 0.0), d.get("wmc", 0.0)
+            k.nb_disabled = d.get("nb_disabled", 0)
+        st = CycleStats()
+        opt = options if options is not None else Options()
+        rc = lib().mcgpu_sync_cycle(self._h, len(ranks), arr, sigs.ctypes.data, ar.ctypes.data,
+                                    weights_absorbed.ctypes.data, len(weights_absorbed),
+                                    nb_particles_per_cycle, nb_particles, max_cycles,
+                                    C.byref(opt), C.byref(st))
+        self.check(rc)
+        out = [{f: getattr(k, f) for f in ("nb_disabled", "n_left", "n_right", "n_killed",
+                                           "n_remaining", "nb_particles_create", "seed")}
+               for k in arr]
+        return out, st.as_dict()
 
     # ------------------------------------------------------------ helpers
     def rng_real_step(self, seeds, reals, stream=None):

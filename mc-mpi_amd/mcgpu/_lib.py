@@ -55,6 +55,25 @@ class Slab(C.Structure):
                 ("particle_offset", C.c_int64), ("n", C.c_int64)]
 
 
+class CycleRank(C.Structure):
+    """include/mcgpu.h mcgpu_cycle_rank: one rank's layer in mcgpu_sync_cycle."""
+    _fields_ = [("min_index", C.c_int32), ("max_index", C.c_int32), ("dx", C.c_double),
+                ("particle_min_weight", C.c_double), ("left_border", C.c_int32),
+                ("right_border", C.c_int32), ("cell_offset", C.c_int64),
+                ("particles", C.c_void_p), ("n_particles", C.c_int64),
+                ("nb_particles_create", C.c_int64), ("seed", C.c_uint64), ("x_ini", C.c_double),
+                ("wmc", C.c_double), ("nb_disabled", C.c_int64), ("n_left", C.c_int64),
+                ("n_right", C.c_int64), ("n_killed", C.c_int64), ("n_remaining", C.c_int64)]
+
+
+class CycleStats(C.Structure):
+    _fields_ = [("cycles", C.c_int64), ("events", C.c_uint64), ("collisions", C.c_uint64),
+                ("kernel_ms", C.c_double), ("wall_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class McgpuError(RuntimeError):
     def __init__(self, status, message):
         super().__init__(f"mcgpu status {status}: {message}")
@@ -95,6 +114,8 @@ SIGNATURES = {
     "mcgpu_math_eval": (i32, [vp, C.c_int, i64, vp, vp, vp]),
     "mcgpu_create_particles": (i32, [vp, u64, i64, i64, C.c_double, C.c_double, C.c_double, vp,
                                      vp]),
+    "mcgpu_sync_cycle": (i32, [vp, i32, C.POINTER(CycleRank), vp, vp, vp, i64, i32, i64, i64,
+                               C.POINTER(Options), C.POINTER(CycleStats)]),
 }
 
 

@@ -2,10 +2,12 @@
 // one GPU) against the oracle's restatement of the synchronous worker loop
 // (orc64_sync_run, src/worker_sync.cpp:24-135).
 //
-// For each case both run the whole loop; then the number of cycles, the
-// device-counted events, and for every rank the particles, escape buffers
-// (bytes), nb_disabled and weights_absorbed (bits; both add each call's
-// exact tally) must be identical.
+// For each case both run the whole loop, SyncCycle once with its
+// device-resident loop (mcgpu_sync_cycle) and once with the host loop over
+// Layer::simulate; then the number of cycles, the device-counted events, and
+// for every rank the particles, escape buffers (bytes), nb_disabled and
+// weights_absorbed (bits; both add each call's exact tally) must be
+// identical.
 #include <stdio.h>
 #include <string.h>
 
@@ -38,7 +40,7 @@ struct Case {
   int kernel;
 };
 
-static void run(const Case &c) {
+static void run(const Case &c, bool device_resident) {
   mcgpu::MCMPIOptions opt{};
   opt.world_size = c.world_size;
   opt.nb_cells = c.nb_cells;
@@ -53,7 +55,9 @@ static void run(const Case &c) {
   opt.statistics_cycle_time = 0.1;
   mcgpu::SyncCycle g(opt);
   for (auto &L : g.layers) L.options.kernel = c.kernel;
+  g.device_resident = device_resident;
   g.spin(10000000);
+  const char *loop = device_resident ? "device loop" : "host loop";
 
   std::vector<orc64_Layer *> O(c.world_size);
   for (int r = 0; r < c.world_size; ++r) {
@@ -63,8 +67,8 @@ static void run(const Case &c) {
   }
   const int64_t cycles = orc64_sync_run(O.data(), c.world_size, c.nb_particles, c.per_cycle, 8,
                                         10000000);
-  CHECK(cycles == g.nb_cycles, "%s cycles %lld vs %lld", c.name, (long long)g.nb_cycles,
-        (long long)cycles);
+  CHECK(cycles == g.nb_cycles, "%s (%s) cycles %lld vs %lld", c.name, loop,
+        (long long)g.nb_cycles, (long long)cycles);
   unsigned long long events = 0;
   long long disabled = 0;
   for (int r = 0; r < c.world_size; ++r) {
@@ -72,22 +76,22 @@ static void run(const Case &c) {
     void *p;
     int64_t n;
     n = orc64_layer_array(O[r], 3, &p);
-    CHECK(same_bytes(G.particles, p, n), "%s rank %d particles", c.name, r);
+    CHECK(same_bytes(G.particles, p, n), "%s (%s) rank %d particles", c.name, loop, r);
     n = orc64_layer_array(O[r], 4, &p);
-    CHECK(same_bytes(G.particles_left, p, n), "%s rank %d particles_left", c.name, r);
+    CHECK(same_bytes(G.particles_left, p, n), "%s (%s) rank %d particles_left", c.name, loop, r);
     n = orc64_layer_array(O[r], 5, &p);
-    CHECK(same_bytes(G.particles_right, p, n), "%s rank %d particles_right", c.name, r);
+    CHECK(same_bytes(G.particles_right, p, n), "%s (%s) rank %d particles_right", c.name, loop, r);
     n = orc64_layer_array(O[r], 2, &p);
-    CHECK(same_bytes(G.weights_absorbed, p, n), "%s rank %d weights_absorbed", c.name, r);
-    CHECK(G.nb_disabled == orc64_layer_int(O[r], 0), "%s rank %d nb_disabled %d vs %lld", c.name,
-          r, G.nb_disabled, (long long)orc64_layer_int(O[r], 0));
+    CHECK(same_bytes(G.weights_absorbed, p, n), "%s (%s) rank %d weights_absorbed", c.name, loop, r);
+    CHECK(G.nb_disabled == orc64_layer_int(O[r], 0), "%s (%s) rank %d nb_disabled %d vs %lld",
+          c.name, loop, r, G.nb_disabled, (long long)orc64_layer_int(O[r], 0));
     events += (unsigned long long)orc64_layer_int(O[r], 7);
     disabled += G.nb_disabled;
     orc64_layer_free(O[r]);
   }
-  CHECK(events == g.events, "%s events %llu vs %llu", c.name, g.events, events);
-  CHECK(disabled == c.nb_particles, "%s disabled %lld", c.name, disabled);
-  printf("ok %s: %lld cycles, %llu events\n", c.name, g.nb_cycles, g.events);
+  CHECK(events == g.events, "%s (%s) events %llu vs %llu", c.name, loop, g.events, events);
+  CHECK(disabled == c.nb_particles, "%s (%s) disabled %lld", c.name, loop, disabled);
+  printf("ok %s (%s): %lld cycles, %llu events\n", c.name, loop, g.nb_cycles, g.events);
   fflush(stdout);
 }
 
@@ -100,9 +104,18 @@ int main() {
       {"uneven split, 7 ranks, source in rank 0", 7, 1000, 10000, 300, 0.05, 1e-6,
        MCGPU_KERNEL_HISTORY},
       {"event kernel, 4 ranks", 4, 1024, 10000, 700, 0.3, 1e-6, MCGPU_KERNEL_EVENT},
+      /* more particles than MAX_PARTICLES_VECTOR: lazy creation in the source
+         rank during the loop */
+      {"lazy creation, 3 ranks, 1.2e6 particles", 3, 99, 1200000, 100000, 0.5, 1e-9,
+       MCGPU_KERNEL_HISTORY},
   };
   try {
-    for (const Case &c : cases) run(c);
+    for (const Case &c : cases) {
+      printf("case %s\n", c.name);
+      fflush(stdout);
+      run(c, true);
+      run(c, false);
+    }
   } catch (const std::exception &e) {
     fprintf(stderr, "exception: %s\n", e.what());
     return 1;
