@@ -12,8 +12,10 @@ Also reported, in "variants":
   event     -- the event-based kernel on the same config-2 bank (one step);
   host_api  -- mcgpu_simulate with a host bank (copies over PCIe included) on
                the first 1e7 particles of config 2;
-  cycle     -- the reference's synchronous multi-rank loop with five layers
-               on this GPU (mcgpu_run --ranks 5), config.yaml source;
+  cycle     -- the reference's headline job (its scripts' config.yaml: 1000
+               cells, 1e8 particles, 1e5 per cycle, sync) with 1 and 8 ranks,
+               every rank on this GPU (mcgpu_run --ranks R, device-resident
+               cycle loop), next to the reference's published wall times;
   batched   -- config 4 (4096 independent slabs, 5.8e8 particles) in one
                persistent launch replayed from a hipGraph, launch shape chosen
                by the autotuner on a quarter of the slabs; and the same
@@ -49,7 +51,7 @@ MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0620, "event": 88.4}
 # VALU wave-instructions per event (rocprofv3 --pmc SQ_INSTS_VALU, same runs).
 # gfx950 issues one wave64 VALU instruction per SIMD every 4 cycles:
 # 256 CUs x 4 SIMDs x 2.4 GHz / 4 = 6.144e11 wave-instructions/s.
-MEASURED_VALU_PER_EVENT = {"history": 2.171, "event": 3.853}
+MEASURED_VALU_PER_EVENT = {"history": 1.799, "event": 3.459}
 VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
 
 
@@ -148,6 +150,31 @@ def cpu_baselines(n_ref, n_port):
     return ref, port
 
 
+def cpu_reference_job(n):
+    """The reference's own Layer (oracle/_ref, FP32 as the reference builds,
+    OpenMP) on the first n particles of its headline job, one rank:
+    decompose_domain + Layer::simulate(-1, threads) does the same histories
+    as the one-rank cycle loop.  Wall seconds, extrapolated to the job's
+    1e8 particles.  None when oracle/_ref is not built."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    if not os.path.exists(pyoracle.REF_PATH):
+        return None
+    threads = min(16, os.cpu_count() or 1)
+    R = pyoracle.RefLayer.decompose_domain(0.0, 1.0, float(np.float32(0.7071067690849304)), 1, 0,
+                                           1000, n, float(np.float32(1e-12)))
+    t0 = time.perf_counter()
+    R.simulate(-1, threads)
+    dt = time.perf_counter() - t0
+    assert R.nb_active == 0
+    res = {"particles": n, "seconds": dt, "threads": threads,
+           "extrapolated_wall_s_1e8": dt * 1e8 / n,
+           "note": "the reference's Layer::simulate(-1, threads) built from src/layer.cpp + "
+                   "src/random.cpp (-O3 -fopenmp, FP32), one rank, this host"}
+    log(f"cpu reference job: {res}")
+    return res
+
+
 def run_plain(ctx, args, torch, mcgpu, configs):
     n = args.particles
     wl = configs.config2(n, with_bank=False)
@@ -215,38 +242,57 @@ def run_host_api(ctx, args, torch, mcgpu, configs):
     return out
 
 
+# The reference's published whole-program wall times for its headline job
+# (scripts/batch.py:55-79: 1000 cells, x_ini = 0.70710677, min weight 1e-12,
+# 1e8 particles, 1e5 per cycle, `mpirun -n R ./main config.yaml sync`, FP32,
+# nthread 1), as recorded in py_data.pkl (si36 cluster, 8 ranks per node).
+REFERENCE_JOB_WALL_S = {1: 2895.78, 2: 1612.93, 4: 952.40, 8: 646.36}
+REFERENCE_JOB_YAML = ("x_min: 0.000000000000000000e+00\nx_max: 1.000000000000000000e+00\n"
+                      "x_ini: 7.071067690849304199e-01\n"
+                      "particle_min_weight: 9.999999960041972003e-13\nnb_cells: 1000\n"
+                      "nb_particles: {n}\ncycle_time: 1.e-05\n"
+                      "nb_particles_per_cycle: 100000\nnthread: 1\nstatistics_cycle_time: 1.e-1\n")
+
+
 def run_cycle(ctx, args, torch, mcgpu, configs):
-    """`mcgpu_run config.yaml --ranks 5` (the synchronous multi-rank loop of
-    the reference's workers, all five layers on this GPU) on the reference's
-    config.yaml source with 1e6 particles and 1e5 per cycle; events / wall
-    seconds of the loop as mcgpu_run reports them."""
+    """The reference's headline job, `mpirun -n R ./main config.yaml sync`
+    with its scripts' config (1000 cells, 1e8 particles, 1e5 per cycle), run
+    by `mcgpu_run config.yaml --ranks R`: every rank's layer on this GPU,
+    the whole cycle loop device-resident.  Wall seconds of the loop as
+    mcgpu_run reports them (src/main.cpp:90-91 prints the same quantity),
+    next to the reference's published wall time for the same job."""
     import subprocess
     import tempfile
     exe = os.path.join(ROOT, "mc-mpi_amd", "mcgpu_run")
-    with tempfile.TemporaryDirectory() as d:
-        cfg = os.path.join(d, "config.yaml")
-        with open(cfg, "w") as f:
-            f.write("x_min: 0.000000000000000000e+00\nx_max: 1.000000000000000000e+00\n"
-                    "x_ini: 7.071067690849304199e-01\n"
-                    "particle_min_weight: 9.999999960041972003e-13\nnb_cells: 1000\n"
-                    f"nb_particles: {args.cycle_particles}\ncycle_time: 1.e-05\n"
-                    "nb_particles_per_cycle: 100000\nnthread: 1\nstatistics_cycle_time: 1.e-1\n")
-        r = subprocess.run([exe, cfg, "--ranks", "5"], cwd=d, capture_output=True, text=True,
-                           timeout=600)
-    if r.returncode != 0:
-        raise RuntimeError(f"mcgpu_run failed: {r.stderr[-500:]}")
-    secs = float(r.stdout.split()[0])
-    # the summary line; a profiler attached to the child adds lines of its own
-    summary = [ln for ln in r.stderr.splitlines() if ln.startswith("events ")]
-    if not summary:
-        raise RuntimeError(f"mcgpu_run printed no summary: {r.stderr[-500:]}")
-    fields = dict(kv.split(" ", 1) for kv in summary[-1].split(", "))
-    events = int(fields["events"])
-    out = {"workload": f"config.yaml source, 1000 cells, {args.cycle_particles} particles, "
-                       "1e5 per cycle, 5 ranks on one GPU (mcgpu_run --ranks 5)",
-           "events": events, "cycles": int(fields["cycles"]), "wall_s": secs,
-           "events_per_s": events / secs}
-    log(f"cycle: {out}")
+    n = args.cycle_particles
+    out = {"workload": f"the reference's headline job: config.yaml of scripts/batch.py, 1000 "
+                       f"cells, {n} particles from x_ini = 0.70710677, min weight 1e-12, 1e5 "
+                       "per cycle, sync; mcgpu_run --ranks R (all ranks on this GPU)",
+           "runs": {}}
+    for R in (int(r) for r in args.cycle_ranks.split(",")):
+        with tempfile.TemporaryDirectory() as d:
+            cfg = os.path.join(d, "config.yaml")
+            with open(cfg, "w") as f:
+                f.write(REFERENCE_JOB_YAML.format(n=n))
+            r = subprocess.run([exe, cfg, "--ranks", str(R)], cwd=d, capture_output=True,
+                               text=True, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(f"mcgpu_run failed: {r.stderr[-500:]}")
+        secs = float(r.stdout.split()[0])
+        # the summary line; a profiler attached to the child adds lines of its own
+        summary = [ln for ln in r.stderr.splitlines() if ln.startswith("events ")]
+        if not summary:
+            raise RuntimeError(f"mcgpu_run printed no summary: {r.stderr[-500:]}")
+        fields = dict(kv.split(" ", 1) for kv in summary[-1].split(", "))
+        events = int(fields["events"])
+        run = {"events": events, "cycles": int(fields["cycles"]), "wall_s": secs,
+               "kernel_s": float(fields["kernel_ms"]) / 1e3, "events_per_s": events / secs}
+        if n == 100_000_000 and R in REFERENCE_JOB_WALL_S:
+            ref = REFERENCE_JOB_WALL_S[R]
+            run["reference_published_wall_s"] = ref
+            run["speedup_vs_published"] = ref / secs
+        out["runs"][str(R)] = run
+        log(f"cycle, {R} ranks: {run}")
     return out
 
 
@@ -405,8 +451,12 @@ def main():
     ap.add_argument("--skip", default="",
                     help="comma-separated variants to leave out (host_api, cycle, batched, "
                          "adjoint, adjoint_batched)")
-    ap.add_argument("--cycle-particles", type=int, default=1_000_000,
-                    help="particles of the multi-rank cycle variant")
+    ap.add_argument("--cycle-particles", type=int, default=100_000_000,
+                    help="particles of the reference-job cycle variant")
+    ap.add_argument("--cycle-ranks", default="1,8",
+                    help="comma-separated rank counts of the cycle variant")
+    ap.add_argument("--cpu-job-sample", type=int, default=4_000_000,
+                    help="particles of the reference CPU leg of the cycle variant")
     ap.add_argument("--host-particles", type=int, default=10_000_000,
                     help="particles of the host-buffer (PCIe-inclusive) variant")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -486,6 +536,10 @@ def main():
         ref, port = cpu_baselines(args.cpu_ref_sample, args.cpu_port_sample)
         out["cpu_baseline"] = ref if ref is not None else port
         out["variants"]["cpu_fp64_port"] = port
+        if "cycle" in out["variants"]:
+            job = cpu_reference_job(args.cpu_job_sample)
+            if job is not None:
+                out["variants"]["cycle"]["reference_cpu_this_host"] = job
     print(json.dumps(out), flush=True)
     ctx.close()
 
