@@ -47,7 +47,7 @@ BYTES_PER_EVENT = 72  # algorithmic: the 36-byte particle state read + written p
 # HBM bytes per event measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
 # this workload (1e7 particles), scaled by the copy-kernel calibration (read
 # x2.0, write x1.0); the counter runs are summarised under profiles/.
-MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0620, "event": 88.4}
+MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0616, "event": 88.4}
 # VALU wave-instructions per event (rocprofv3 --pmc SQ_INSTS_VALU, same runs).
 # gfx950 issues one wave64 VALU instruction per SIMD every 4 cycles:
 # 256 CUs x 4 SIMDs x 2.4 GHz / 4 = 6.144e11 wave-instructions/s.
