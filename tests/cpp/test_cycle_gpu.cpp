@@ -108,6 +108,14 @@ int main() {
          rank during the loop */
       {"lazy creation, 3 ranks, 1.2e6 particles", 3, 99, 1200000, 100000, 0.5, 1e-9,
        MCGPU_KERNEL_HISTORY},
+      /* many ranks, tiny cycles: 16 layers of 62-63 cells, 7 particles per
+         cycle and rank */
+      {"16 ranks, 7 per cycle", 16, 1000, 2000, 7, 0.41, 1e-7, MCGPU_KERNEL_HISTORY},
+      /* the source in the last rank (the right border) */
+      {"source in the right-border rank, 6 ranks", 6, 600, 8000, 250, 0.97, 1e-7,
+       MCGPU_KERNEL_HISTORY},
+      /* kills dominate: weights 1e-4 against a minimum of 5e-5 */
+      {"kills dominate, 4 ranks", 4, 400, 10000, 333, 0.6, 5e-5, MCGPU_KERNEL_HISTORY},
   };
   try {
     for (const Case &c : cases) {
