@@ -67,6 +67,7 @@ struct mc_cell {
   double sig_a;
   double sig_i;
 };
+static_assert(sizeof(mc_cell) == 16, "cell offsets are formed as c << 4");
 
 __device__ __forceinline__ mc_cell mc_make_cell(double sigs, double ar) {
   const double interaction_rate = 1.0 - ar;
@@ -87,6 +88,17 @@ __device__ __forceinline__ void mc_fx_atomic_add(unsigned long long *lo_word,
   unsigned long long carry = (old + v.lo < old) ? 1ULL : 0ULL;
   unsigned long long h = v.hi + carry;
   if (h) atomicAdd(hi_word, h);
+}
+
+/* The same without tests: both words are added even when a part is zero
+   (adding zero changes nothing).  The transport's deposits are almost never
+   zero, and the tests and their branches cost more than the rare extra
+   atomic. */
+__device__ __forceinline__ void mc_fx_atomic_add_always(unsigned long long *lo_word,
+                                                        unsigned long long *hi_word, mc_fx v) {
+  const unsigned long long old = atomicAdd(lo_word, (unsigned long long)v.lo);
+  const unsigned long long carry = (old + v.lo < old) ? 1ULL : 0ULL;
+  atomicAdd(hi_word, (unsigned long long)v.hi + carry);
 }
 
 /* --------------------------------------------------------- particle state */
@@ -147,7 +159,10 @@ __device__ __forceinline__ double mc_event(mc_state &p, const mc_cell *cells, in
   /* The edge quotient is formed unconditionally and then selected; when
      |mu| <= EPS the discarded quotient may be inf or NaN and is never used.
      *cell_out (if given) receives the cell. */
-  const mc_cell cell = cells[c];
+  /* an unsigned 32-bit byte offset (c is in [0, m)): the load takes the
+     uniform base in scalar registers and the offset as is */
+  const mc_cell cell =
+      *reinterpret_cast<const mc_cell *>(reinterpret_cast<const char *>(cells) + ((uint32_t)c << 4));
   if (cell_out) *cell_out = cell;
   const double h = mc_rnd_real(p.seed);
 
@@ -240,7 +255,10 @@ __device__ __forceinline__ void mc_tally(unsigned long long *lo, unsigned long l
       if (g.flags & 8u) atomicOr(status, 2u);
     }
   }
-  mc_fx_atomic_add(&lo[c], &hi[c], v);
+  if (SIGNED)
+    mc_fx_atomic_add(&lo[c], &hi[c], v);
+  else
+    mc_fx_atomic_add_always(&lo[c], &hi[c], v);
 }
 
 /* Gradient terms of one replayed event for the target T = sum g_c tally_c
