@@ -32,6 +32,13 @@
 namespace {
 
 #define DEAD_ID 0xFFFFFFFFu
+/* Bit 31 of a live slot's id: the particle's last event crossed a cell edge,
+   so its x is the edge its cell index and direction imply,
+   (mu < 0 ? index + 1 : index) * dx -- the very product the crossing stored
+   (src/layer.cpp:141-143,168-171) -- and the slot's x field is neither
+   written nor read.  Most events are crossings, so a pass moves 36 bytes per
+   particle each way instead of 44.  Particle ids stay below 2^31 - 1. */
+#define ON_EDGE 0x80000000u
 
 /* A block's pool: two halves of P slots each, structure of arrays.  Field
    k of slot j in half h lives at base + h*HALF + OFF_k*P + j*size_k, with
@@ -79,13 +86,13 @@ template <typename T> __device__ __forceinline__ void st_nt(T *p, T v) {
 
 template <bool COLL>
 __device__ __forceinline__ void put(const pool_ref &h, int j, const mc_state &p, uint32_t id,
-                                    uint32_t ev, uint32_t col) {
+                                    uint32_t ev, uint32_t col, bool on_edge) {
   st_nt(h.seed() + j, p.seed);
-  st_nt(h.x() + j, p.x);
+  if (!on_edge) st_nt(h.x() + j, p.x);
   st_nt(h.mu() + j, p.mu);
   st_nt(h.wmc() + j, p.wmc);
   st_nt(h.index() + j, p.index);
-  st_nt(h.id() + j, id);
+  st_nt(h.id() + j, on_edge ? (id | ON_EDGE) : id);
   st_nt(h.ev() + j, ev);
   if (COLL) st_nt(h.col() + j, col);
 }
@@ -98,15 +105,20 @@ struct slot_regs {
 };
 
 template <bool COLL>
-__device__ __forceinline__ void load_slot(const pool_ref &h, int j, slot_regs &r) {
+__device__ __forceinline__ void load_slot(const pool_ref &h, int j, double dx, slot_regs &r) {
   r.id = ld_nt(h.id() + j);
   r.p.seed = ld_nt(h.seed() + j);
-  r.p.x = ld_nt(h.x() + j);
   r.p.mu = ld_nt(h.mu() + j);
   r.p.wmc = ld_nt(h.wmc() + j);
   r.p.index = ld_nt(h.index() + j);
   r.ev = ld_nt(h.ev() + j);
   r.col = COLL ? ld_nt(h.col() + j) : 0u;
+  if (r.id != DEAD_ID && (r.id & ON_EDGE)) {
+    r.id &= ~ON_EDGE;
+    r.p.x = (r.p.mu < 0 ? r.p.index + 1 : r.p.index) * dx;
+  } else if (r.id != DEAD_ID) {
+    r.p.x = ld_nt(h.x() + j);
+  }
 }
 
 template <bool LDS_TALLY, bool LDS_CELLS, bool COLL>
@@ -171,7 +183,7 @@ __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned ch
       for (int k = lane; k < take; k += 64) {
         mc_state p;
         load_state(a, (int64_t)(base + k), p);
-        put<COLL>(h, n_pool + k, p, (uint32_t)(base + k), 0u, 0u);
+        put<COLL>(h, n_pool + k, p, (uint32_t)(base + k), 0u, 0u, false);
       }
       n_pool += take;
       /* slots written by one lane are read by another lane of this wave */
@@ -192,23 +204,25 @@ __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned ch
        computed, so its memory latency overlaps the FP64 work */
     slot_regs nxt;
     nxt.valid = lane < n_pool;
-    if (nxt.valid) load_slot<COLL>(src, lane, nxt);
+    if (nxt.valid) load_slot<COLL>(src, lane, a.L.dx, nxt);
     for (int j0 = 0; j0 < n_pool; j0 += 64) { /* wave-uniform trip count */
       const int j = j0 + lane;
       mc_state p = nxt.p;
       uint32_t id = nxt.valid ? nxt.id : DEAD_ID, ev = nxt.ev, col = nxt.col;
       if (j0 + 64 < n_pool) {
         nxt.valid = j + 64 < n_pool;
-        if (nxt.valid) load_slot<COLL>(src, j + 64, nxt);
+        if (nxt.valid) load_slot<COLL>(src, j + 64, a.L.dx, nxt);
       }
       const bool live = id != DEAD_ID;
       bool keep = false;
+      bool on_edge = false; /* set by a crossing */
       if (live) {
         if (mc_alive(p, a.L) && ev < max_ev) {
           const int c = p.index - a.L.index_start;
           bool coll;
           double di;
           const double dw = mc_event(p, cells, c, a.L.dx, K, coll, di);
+          on_edge = !coll;
           ++ev;
           ++pass_ev;
           col += coll ? 1u : 0u;
@@ -231,11 +245,11 @@ __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned ch
       if (compact) {
         /* stream compaction: survivors keep their order, ranks from the ballot */
         const unsigned long long mask = __ballot(keep);
-        if (keep) put<COLL>(dst, wpos + mc_mbcnt64(mask), p, id, ev, col);
+        if (keep) put<COLL>(dst, wpos + mc_mbcnt64(mask), p, id, ev, col, on_edge);
         wpos += __popcll(mask);
       } else if (live) {
         if (keep)
-          put<COLL>(src, j, p, id, ev, col);
+          put<COLL>(src, j, p, id, ev, col, on_edge);
         else
           st_nt(src.id() + j, DEAD_ID);
       }
@@ -313,8 +327,8 @@ int launch_one(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_optio
 
 int mc_launch_event(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_options &o,
                     hipStream_t s) {
-  if (a.n >= (int64_t)DEAD_ID)
-    return mc_fail(ctx, MCGPU_ERR_ARG, "event kernel: bank larger than 2^32-2 particles");
+  if (a.n >= (int64_t)(ON_EDGE - 1))
+    return mc_fail(ctx, MCGPU_ERR_ARG, "event kernel: bank larger than 2^31-2 particles");
   const size_t m = (size_t)a.L.m;
   const size_t tally = 16 * m, cells = sizeof(mc_cell) * m;
   const bool lds_tally = tally <= ctx->lds_per_block;
