@@ -17,8 +17,9 @@
  *    ballot + mbcnt prefix gives each survivor its rank, one LDS atomic per
  *    wave reserves the wave's range; finished histories are written to their
  *    original bank slot (the particle id), so compaction order is free;
- *  - with compaction interval c > 1, c-1 passes out of c write survivors back
- *    in place and mark finished slots dead instead of compacting;
+ *  - with compaction interval c > 1 (default 16), c-1 passes out of c write
+ *    survivors back in place -- only the fields their event changed -- and
+ *    mark finished slots dead instead of compacting;
  *  - when the pool has room for a block-width of particles, it is refilled
  *    from the call's bank through the global work queue (one atomic per
  *    refill), so blocks stay full until the bank is exhausted;
@@ -97,10 +98,31 @@ __device__ __forceinline__ void put(const pool_ref &h, int j, const mc_state &p,
   if (COLL) st_nt(h.col() + j, col);
 }
 
+/* An in-place pass (no compaction) writes back only what the event changed:
+   seed, wmc and the event counter always; the cell index after a crossing;
+   x and mu after a collision; the id when the on-edge flag changed. */
+template <bool COLL>
+__device__ __forceinline__ void put_changed(const pool_ref &h, int j, const mc_state &p,
+                                            uint32_t id, uint32_t ev, uint32_t col,
+                                            bool on_edge, bool was_on_edge) {
+  st_nt(h.seed() + j, p.seed);
+  st_nt(h.wmc() + j, p.wmc);
+  st_nt(h.ev() + j, ev);
+  if (COLL) st_nt(h.col() + j, col);
+  if (on_edge) {
+    st_nt(h.index() + j, p.index);
+  } else {
+    st_nt(h.x() + j, p.x);
+    st_nt(h.mu() + j, p.mu);
+  }
+  if (on_edge != was_on_edge) st_nt(h.id() + j, on_edge ? (id | ON_EDGE) : id);
+}
+
 /* the registers of one pool slot */
 struct slot_regs {
   mc_state p;
   uint32_t id, ev, col;
+  bool on_edge; /* the slot's x was implicit */
   bool valid;
 };
 
@@ -113,7 +135,8 @@ __device__ __forceinline__ void load_slot(const pool_ref &h, int j, double dx, s
   r.p.index = ld_nt(h.index() + j);
   r.ev = ld_nt(h.ev() + j);
   r.col = COLL ? ld_nt(h.col() + j) : 0u;
-  if (r.id != DEAD_ID && (r.id & ON_EDGE)) {
+  r.on_edge = r.id != DEAD_ID && (r.id & ON_EDGE);
+  if (r.on_edge) {
     r.id &= ~ON_EDGE;
     r.p.x = (r.p.mu < 0 ? r.p.index + 1 : r.p.index) * dx;
   } else if (r.id != DEAD_ID) {
@@ -209,6 +232,7 @@ __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned ch
       const int j = j0 + lane;
       mc_state p = nxt.p;
       uint32_t id = nxt.valid ? nxt.id : DEAD_ID, ev = nxt.ev, col = nxt.col;
+      const bool was_on_edge = nxt.valid && nxt.on_edge;
       if (j0 + 64 < n_pool) {
         nxt.valid = j + 64 < n_pool;
         if (nxt.valid) load_slot<COLL>(src, j + 64, a.L.dx, nxt);
@@ -249,7 +273,7 @@ __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned ch
         wpos += __popcll(mask);
       } else if (live) {
         if (keep)
-          put<COLL>(src, j, p, id, ev, col, on_edge);
+          put_changed<COLL>(src, j, p, id, ev, col, on_edge, was_on_edge);
         else
           st_nt(src.id() + j, DEAD_ID);
       }
@@ -314,7 +338,10 @@ int launch_one(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_optio
     grid = (int)((a.n + block - 1) / block);
     if (grid < 1) grid = 1;
   }
-  const int c = o.compaction_interval > 0 ? o.compaction_interval : 1;
+  /* compaction every 16th pass: the passes between write back only what
+     changed (measured on config 2: every pass 10.13 s, every 4th 9.25 s,
+     8th 9.11 s, 16th 9.04 s, 32nd 9.02 s) */
+  const int c = o.compaction_interval > 0 ? o.compaction_interval : 16;
   const size_t waves = (size_t)grid * (size_t)(block / 64);
   int rc = mc_ensure(ctx, &ctx->work, &ctx->work_bytes, waves * 2 * SLOT_BYTES * (size_t)P);
   if (rc) return rc;

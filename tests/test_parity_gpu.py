@@ -195,7 +195,8 @@ LAUNCH_OPTS = [
     dict(kernel="history", grid_blocks=1), dict(kernel="history", grid_blocks=3, block_size=128),
     dict(kernel="event", block_size=64), dict(kernel="event", block_size=1024),
     dict(kernel="event", particles_per_lane=1), dict(kernel="event", particles_per_lane=8),
-    dict(kernel="event", compaction_interval=3), dict(kernel="event", compaction_interval=16),
+    dict(kernel="event", compaction_interval=1), dict(kernel="event", compaction_interval=3),
+    dict(kernel="event", compaction_interval=16), dict(kernel="event", compaction_interval=64),
     dict(kernel="event", grid_blocks=1), dict(kernel="event", grid_blocks=5, block_size=128,
                                               particles_per_lane=2, compaction_interval=2)]
 
