@@ -9,9 +9,9 @@
  *
  * Layout and schedule (one persistent block per CU-slot, no launch per pass):
  *  - each block owns a particle pool in HBM: two halves (ping-pong) of P
- *    slots in structure-of-arrays form (seed, x, mu, wmc, index, id, event
- *    and collision counters), so every field is a coalesced 512-byte wave
- *    access;
+ *    slots in structure-of-arrays form (seed, x, mu, wmc, index, id, the
+ *    pass its event count started from, collision counter), so every field
+ *    is a coalesced 512-byte wave access;
  *  - a pass streams the live part of the current half: load, one event,
  *    tally, then the survivors are compacted into the other half -- a wave
  *    ballot + mbcnt prefix gives each survivor its rank, one LDS atomic per
@@ -56,7 +56,9 @@ struct pool_ref {
   __device__ __forceinline__ double *wmc() const { return reinterpret_cast<double *>(b + 24 * (size_t)P); }
   __device__ __forceinline__ int32_t *index() const { return reinterpret_cast<int32_t *>(b + 32 * (size_t)P); }
   __device__ __forceinline__ uint32_t *id() const { return reinterpret_cast<uint32_t *>(b + 36 * (size_t)P); }
-  __device__ __forceinline__ uint32_t *ev() const { return reinterpret_cast<uint32_t *>(b + 40 * (size_t)P); }
+  /* the pass at which the particle's event count was 0: every pass gives
+     every live slot one event, so the count before pass k is k - t0 */
+  __device__ __forceinline__ uint32_t *t0() const { return reinterpret_cast<uint32_t *>(b + 40 * (size_t)P); }
   __device__ __forceinline__ uint32_t *col() const { return reinterpret_cast<uint32_t *>(b + 44 * (size_t)P); }
 };
 
@@ -87,27 +89,26 @@ template <typename T> __device__ __forceinline__ void st_nt(T *p, T v) {
 
 template <bool COLL>
 __device__ __forceinline__ void put(const pool_ref &h, int j, const mc_state &p, uint32_t id,
-                                    uint32_t ev, uint32_t col, bool on_edge) {
+                                    uint32_t t0, uint32_t col, bool on_edge) {
   st_nt(h.seed() + j, p.seed);
   if (!on_edge) st_nt(h.x() + j, p.x);
   st_nt(h.mu() + j, p.mu);
   st_nt(h.wmc() + j, p.wmc);
   st_nt(h.index() + j, p.index);
   st_nt(h.id() + j, on_edge ? (id | ON_EDGE) : id);
-  st_nt(h.ev() + j, ev);
+  st_nt(h.t0() + j, t0);
   if (COLL) st_nt(h.col() + j, col);
 }
 
 /* An in-place pass (no compaction) writes back only what the event changed:
-   seed, wmc and the event counter always; the cell index after a crossing;
-   x and mu after a collision; the id when the on-edge flag changed. */
+   seed and wmc always; the cell index after a crossing; x and mu after a
+   collision; the id when the on-edge flag changed.  (t0 stays.) */
 template <bool COLL>
 __device__ __forceinline__ void put_changed(const pool_ref &h, int j, const mc_state &p,
-                                            uint32_t id, uint32_t ev, uint32_t col,
-                                            bool on_edge, bool was_on_edge) {
+                                            uint32_t id, uint32_t col, bool on_edge,
+                                            bool was_on_edge) {
   st_nt(h.seed() + j, p.seed);
   st_nt(h.wmc() + j, p.wmc);
-  st_nt(h.ev() + j, ev);
   if (COLL) st_nt(h.col() + j, col);
   if (on_edge) {
     st_nt(h.index() + j, p.index);
@@ -127,13 +128,14 @@ struct slot_regs {
 };
 
 template <bool COLL>
-__device__ __forceinline__ void load_slot(const pool_ref &h, int j, double dx, slot_regs &r) {
+__device__ __forceinline__ void load_slot(const pool_ref &h, int j, double dx, uint32_t pass,
+                                          slot_regs &r) {
   r.id = ld_nt(h.id() + j);
   r.p.seed = ld_nt(h.seed() + j);
   r.p.mu = ld_nt(h.mu() + j);
   r.p.wmc = ld_nt(h.wmc() + j);
   r.p.index = ld_nt(h.index() + j);
-  r.ev = ld_nt(h.ev() + j);
+  r.ev = pass - ld_nt(h.t0() + j);
   r.col = COLL ? ld_nt(h.col() + j) : 0u;
   r.on_edge = r.id != DEAD_ID && (r.id & ON_EDGE);
   if (r.on_edge) {
@@ -206,7 +208,7 @@ __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned ch
       for (int k = lane; k < take; k += 64) {
         mc_state p;
         load_state(a, (int64_t)(base + k), p);
-        put<COLL>(h, n_pool + k, p, (uint32_t)(base + k), 0u, 0u, false);
+        put<COLL>(h, n_pool + k, p, (uint32_t)(base + k), (uint32_t)pass, 0u, false);
       }
       n_pool += take;
       /* slots written by one lane are read by another lane of this wave */
@@ -227,7 +229,7 @@ __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned ch
        computed, so its memory latency overlaps the FP64 work */
     slot_regs nxt;
     nxt.valid = lane < n_pool;
-    if (nxt.valid) load_slot<COLL>(src, lane, a.L.dx, nxt);
+    if (nxt.valid) load_slot<COLL>(src, lane, a.L.dx, pass, nxt);
     for (int j0 = 0; j0 < n_pool; j0 += 64) { /* wave-uniform trip count */
       const int j = j0 + lane;
       mc_state p = nxt.p;
@@ -235,7 +237,7 @@ __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned ch
       const bool was_on_edge = nxt.valid && nxt.on_edge;
       if (j0 + 64 < n_pool) {
         nxt.valid = j + 64 < n_pool;
-        if (nxt.valid) load_slot<COLL>(src, j + 64, a.L.dx, nxt);
+        if (nxt.valid) load_slot<COLL>(src, j + 64, a.L.dx, pass, nxt);
       }
       const bool live = id != DEAD_ID;
       bool keep = false;
@@ -269,11 +271,11 @@ __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned ch
       if (compact) {
         /* stream compaction: survivors keep their order, ranks from the ballot */
         const unsigned long long mask = __ballot(keep);
-        if (keep) put<COLL>(dst, wpos + mc_mbcnt64(mask), p, id, ev, col, on_edge);
+        if (keep) put<COLL>(dst, wpos + mc_mbcnt64(mask), p, id, pass + 1u - ev, col, on_edge);
         wpos += __popcll(mask);
       } else if (live) {
         if (keep)
-          put_changed<COLL>(src, j, p, id, ev, col, on_edge, was_on_edge);
+          put_changed<COLL>(src, j, p, id, col, on_edge, was_on_edge);
         else
           st_nt(src.id() + j, DEAD_ID);
       }
