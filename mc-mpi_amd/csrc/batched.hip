@@ -51,7 +51,7 @@ struct batch_lane {
 /* One tile: its histories, lanes refilled from the LDS queue s_next.  The
    tally and cell pointers are LDS or global according to the template
    arguments, so every access compiles to the matching instruction. */
-template <bool LDS_TALLY, bool LDS_CELLS>
+template <bool LDS_TALLY, bool LDS_CELLS, bool HI_COND>
 __device__ __forceinline__ void batch_tile(const batch_args &a, const mc_tile_k &tile,
                                            const mc_slab_k &sl, const mc_layer_k &L,
                                            unsigned long long *t_lo, unsigned long long *t_hi,
@@ -101,7 +101,7 @@ __device__ __forceinline__ void batch_tile(const batch_args &a, const mc_tile_k 
         const double dw = mc_event<true>(p, cells, c, L.dx, K, coll, di);
         ++ev;
         col += coll ? 1u : 0u;
-        mc_tally(t_lo, t_hi, a.flags + sl.cell_off, &a.ctr->status, c, dw);
+        mc_tally<false, HI_COND>(t_lo, t_hi, a.flags + sl.cell_off, &a.ctr->status, c, dw);
       }
       const bool alive = mc_alive(p, L);
       if (!alive || ev >= max_ev) {
@@ -130,6 +130,8 @@ __device__ __forceinline__ void batch_tile(const batch_args &a, const mc_tile_k 
   }
 }
 
+/* HI_COND: every particle weight is below 2^-24 (see mc_fx_atomic_add_always) */
+template <bool HI_COND>
 __global__ __launch_bounds__(1024) void k_batched(batch_args a, int refill_k) {
   extern __shared__ __align__(16) unsigned char smem[];
   const mc_coef K = mc_coef_live();
@@ -173,13 +175,13 @@ __global__ __launch_bounds__(1024) void k_batched(batch_args a, int refill_k) {
     __syncthreads();
 
     if (lds_cells)
-      batch_tile<true, true>(a, tile, sl, L, t_lds, t_lds + m, lc, &s_next, s_tot, max_ev,
+      batch_tile<true, true, HI_COND>(a, tile, sl, L, t_lds, t_lds + m, lc, &s_next, s_tot, max_ev,
                              refill_k, K, bl);
     else if (lds_tally)
-      batch_tile<true, false>(a, tile, sl, L, t_lds, t_lds + m, a.cells + sl.cell_off, &s_next,
+      batch_tile<true, false, HI_COND>(a, tile, sl, L, t_lds, t_lds + m, a.cells + sl.cell_off, &s_next,
                               s_tot, max_ev, refill_k, K, bl);
     else
-      batch_tile<false, false>(a, tile, sl, L, a.fx + sl.cell_off,
+      batch_tile<false, false, HI_COND>(a, tile, sl, L, a.fx + sl.cell_off,
                                a.fx + a.total_cells + sl.cell_off, a.cells + sl.cell_off,
                                &s_next, s_tot, max_ev, refill_k, K, bl);
 
@@ -257,12 +259,16 @@ int mc_batched_plan_shape(mcgpu_context *ctx, int max_m, int block, int grid, in
   sh.lds_cells_max_m = (int)(sh.lds / 32);
   sh.lds_tally_max_m = (int)(sh.lds / 16);
   if (sh.lds > 65536)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_batched),
+  {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_batched<false>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.lds);
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(k_batched<true>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)sh.lds);
+  }
   sh.grid = grid;
   if (sh.grid <= 0) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batched, block, sh.lds) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batched<false>, block, sh.lds) !=
             hipSuccess ||
         per_cu < 1)
       per_cu = 1;
@@ -276,7 +282,7 @@ int mc_launch_batched(mcgpu_context *ctx, const mc_batched_shape &sh, const mc_s
                       const mc_tile_k *tiles, int32_t n_tiles, const mcgpu_bank &bank,
                       const mc_cell *cells, unsigned long long *fx, int64_t total_cells,
                       uint32_t *flags, int8_t *result8, mc_counters *ctr, uint64_t max_events,
-                      hipStream_t s) {
+                      bool small_weights, hipStream_t s) {
   batch_args a;
   a.slabs = slabs;
   a.tiles = tiles;
@@ -291,7 +297,10 @@ int mc_launch_batched(mcgpu_context *ctx, const mc_batched_shape &sh, const mc_s
   a.result8 = result8;
   a.ctr = ctr;
   a.max_events = max_events;
-  hipLaunchKernelGGL(k_batched, dim3(sh.grid), dim3(sh.block), sh.lds, s, a, sh.refill);
+  if (small_weights)
+    hipLaunchKernelGGL(k_batched<true>, dim3(sh.grid), dim3(sh.block), sh.lds, s, a, sh.refill);
+  else
+    hipLaunchKernelGGL(k_batched<false>, dim3(sh.grid), dim3(sh.block), sh.lds, s, a, sh.refill);
   MC_HIP(ctx, hipGetLastError());
   return MCGPU_OK;
 }
@@ -407,7 +416,8 @@ int batch_capture(mcgpu_batch *b) {
   MC_HIP(ctx, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   hipLaunchKernelGGL(k_batch_setup, dim3((unsigned)((tc + 255) / 256)), dim3(256), 0, b->stream,
                      tc, b->sigs, b->ar, b->d_cells, b->d_fx, b->d_flags, b->d_ctr);
-  hipLaunchKernelGGL(k_batched, dim3(sh.grid), dim3(sh.block), sh.lds, b->stream, a, sh.refill);
+  hipLaunchKernelGGL(k_batched<false>, dim3(sh.grid), dim3(sh.block), sh.lds, b->stream, a,
+                     sh.refill);
   hipLaunchKernelGGL(k_batch_finalize, dim3((unsigned)((tc + 255) / 256)), dim3(256), 0,
                      b->stream, tc, (const unsigned long long *)b->d_fx,
                      (const uint32_t *)b->d_flags, b->weights);

@@ -36,6 +36,7 @@
  */
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <vector>
 
 #include "internal.h"
@@ -512,6 +513,11 @@ extern "C" int mcgpu_sync_cycle(mcgpu_context *ctx, int32_t world_size, mcgpu_cy
     cap = new_cap;
     return MCGPU_OK;
   };
+  /* weights only decrease; when every initial one is below 2^-24 the
+     tally's high words only take carries (mc_launch_batched, small_weights) */
+  double wmax = 0.0;
+  for (int r = 0; r < R; ++r)
+    if (ranks[r].nb_particles_create > 0) wmax = std::max(wmax, std::fabs(ranks[r].wmc));
   {
     std::vector<int64_t> c0(R);
     for (int r = 0; r < R; ++r) c0[r] = need_of(r);
@@ -530,6 +536,7 @@ extern "C" int mcgpu_sync_cycle(mcgpu_context *ctx, int32_t world_size, mcgpu_cy
         hm[i] = q.mu;
         hw[i] = q.wmc;
         hi[i] = q.index;
+        wmax = std::max(wmax, std::fabs(q.wmc));
       }
       const int64_t b = base[r];
       MC_HIP(ctx, hipMemcpyAsync(pool.p.seed + b, hs.data(), 8 * n, hipMemcpyHostToDevice, s));
@@ -614,7 +621,8 @@ extern "C" int mcgpu_sync_cycle(mcgpu_context *ctx, int32_t world_size, mcgpu_cy
     MC_HIP(ctx, hipEventRecord(ctx->ev0, s));
     if (n_tiles > 0 &&
         (rc = mc_launch_batched(ctx, sh, d_slabs, d_tiles, (int32_t)n_tiles, bank, d_cells, d_fx,
-                                tc, d_flags, pool.result8, d_ctr, max_events, s)))
+                                tc, d_flags, pool.result8, d_ctr, max_events,
+                                wmax < 0x1p-24, s)))
       return rc;
     MC_HIP(ctx, hipEventRecord(ctx->ev1, s));
     hipLaunchKernelGGL(k_cyc_finalize, dim3(blocks_for(tc, 256)), dim3(256), 0, s, tc,

@@ -155,11 +155,12 @@ struct mc_batched_shape {
 int mc_batched_plan_shape(mcgpu_context *ctx, int max_m, int block, int grid, int refill,
                           mc_batched_shape *out);
 /* One launch over n_tiles tiles; fx / flags / ctr must be zeroed first
-   (ctr->next is the tile queue head). */
+   (ctr->next is the tile queue head).  small_weights: every particle weight
+   is below 2^-24 (a faster tally add; results are the same either way). */
 int mc_launch_batched(mcgpu_context *ctx, const mc_batched_shape &sh, const mc_slab_k *slabs,
                       const mc_tile_k *tiles, int32_t n_tiles, const mcgpu_bank &bank,
                       const mc_cell *cells, unsigned long long *fx, int64_t total_cells,
                       uint32_t *flags, int8_t *result8, mc_counters *ctr, uint64_t max_events,
-                      hipStream_t s);
+                      bool small_weights, hipStream_t s);
 
 #endif

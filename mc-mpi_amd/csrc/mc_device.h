@@ -90,15 +90,19 @@ __device__ __forceinline__ void mc_fx_atomic_add(unsigned long long *lo_word,
   if (h) atomicAdd(hi_word, h);
 }
 
-/* The same without tests: both words are added even when a part is zero
-   (adding zero changes nothing).  The transport's deposits are almost never
-   zero, and the tests and their branches cost more than the rare extra
-   atomic. */
+/* The same without the test of the whole term: both words are added even
+   when zero (adding zero changes nothing; the transport's deposits are
+   almost never zero, and the tests and their branches cost more than the
+   rare extra atomic).  HI_COND: the high word only when it changes -- for
+   particle weights below 2^-24, whose deposits add nothing there but their
+   carries; then the skipped atomic costs more than the test. */
+template <bool HI_COND>
 __device__ __forceinline__ void mc_fx_atomic_add_always(unsigned long long *lo_word,
                                                         unsigned long long *hi_word, mc_fx v) {
   const unsigned long long old = atomicAdd(lo_word, (unsigned long long)v.lo);
   const unsigned long long carry = (old + v.lo < old) ? 1ULL : 0ULL;
-  atomicAdd(hi_word, (unsigned long long)v.hi + carry);
+  const unsigned long long h = (unsigned long long)v.hi + carry;
+  if (!HI_COND || h) atomicAdd(hi_word, h);
 }
 
 /* --------------------------------------------------------- particle state */
@@ -241,7 +245,7 @@ __device__ __forceinline__ unsigned long long mc_wave_sum(unsigned long long v) 
 /* SIGNED: terms of either sign are common (adjoint); otherwise negative terms
    take the general conversion (same result, fewer instructions for the
    transport's non-negative deposits). */
-template <bool SIGNED = false>
+template <bool SIGNED = false, bool HI_COND = false>
 __device__ __forceinline__ void mc_tally(unsigned long long *lo, unsigned long long *hi,
                                          uint32_t *flags, unsigned int *status, int c, double dw) {
   mc_fx v;
@@ -258,7 +262,7 @@ __device__ __forceinline__ void mc_tally(unsigned long long *lo, unsigned long l
   if (SIGNED)
     mc_fx_atomic_add(&lo[c], &hi[c], v);
   else
-    mc_fx_atomic_add_always(&lo[c], &hi[c], v);
+    mc_fx_atomic_add_always<HI_COND>(&lo[c], &hi[c], v);
 }
 
 /* Gradient terms of one replayed event for the target T = sum g_c tally_c

@@ -38,7 +38,14 @@ struct Case {
   int world_size, nb_cells, nb_particles, per_cycle;
   double x_ini, min_w;
   int kernel;
+  double wscale = 1.0; /* initial weights and the minimum weight scaled by this */
 };
+
+/* every particle's weight times s (all particles are created up front when
+   nb_particles <= MAX_PARTICLES_VECTOR) */
+static void scale_weights(mcgpu_particle *p, int64_t n, double s) {
+  for (int64_t i = 0; i < n; ++i) p[i].wmc *= s;
+}
 
 static void run(const Case &c, bool device_resident) {
   mcgpu::MCMPIOptions opt{};
@@ -47,14 +54,17 @@ static void run(const Case &c, bool device_resident) {
   opt.x_min = 0.0;
   opt.x_max = 1.0;
   opt.x_ini = c.x_ini;
-  opt.particle_min_weight = c.min_w;
+  opt.particle_min_weight = c.min_w * c.wscale;
   opt.nb_particles = c.nb_particles;
   opt.cycle_time = 1e-5;
   opt.nb_particles_per_cycle = c.per_cycle;
   opt.nthread = 1;
   opt.statistics_cycle_time = 0.1;
   mcgpu::SyncCycle g(opt);
-  for (auto &L : g.layers) L.options.kernel = c.kernel;
+  for (auto &L : g.layers) {
+    L.options.kernel = c.kernel;
+    scale_weights(L.particles.data(), (int64_t)L.particles.size(), c.wscale);
+  }
   g.device_resident = device_resident;
   g.spin(10000000);
   const char *loop = device_resident ? "device loop" : "host loop";
@@ -62,8 +72,11 @@ static void run(const Case &c, bool device_resident) {
   std::vector<orc64_Layer *> O(c.world_size);
   for (int r = 0; r < c.world_size; ++r) {
     O[r] = orc64_decompose_domain(0.0, 1.0, c.x_ini, c.world_size, r, c.nb_cells, c.nb_particles,
-                                  c.min_w);
+                                  c.min_w * c.wscale);
     orc64_layer_set_tally_exact(O[r], 1);
+    void *bank;
+    const int64_t nb = orc64_layer_array(O[r], 3, &bank);
+    scale_weights(static_cast<mcgpu_particle *>(bank), nb, c.wscale);
   }
   const int64_t cycles = orc64_sync_run(O.data(), c.world_size, c.nb_particles, c.per_cycle, 8,
                                         10000000);
@@ -116,6 +129,10 @@ int main() {
        MCGPU_KERNEL_HISTORY},
       /* kills dominate: weights 1e-4 against a minimum of 5e-5 */
       {"kills dominate, 4 ranks", 4, 400, 10000, 333, 0.6, 5e-5, MCGPU_KERNEL_HISTORY},
+      /* weights below 2^-24 (the device loop's tally then adds the high
+         words only for carries) */
+      {"weights of 5e-9, 5 ranks", 5, 1000, 20000, 500, 7.071067690849304199e-01,
+       9.999999960041972003e-13, MCGPU_KERNEL_HISTORY, 1e-4},
   };
   try {
     for (const Case &c : cases) {
