@@ -47,11 +47,11 @@ BYTES_PER_EVENT = 72  # algorithmic: the 36-byte particle state read + written p
 # HBM bytes per event measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
 # this workload (1e7 particles), scaled by the copy-kernel calibration (read
 # x2.0, write x1.0); the counter runs are summarised under profiles/.
-MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0616, "event": 88.4}
+MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0616, "event": 55.7}
 # VALU wave-instructions per event (rocprofv3 --pmc SQ_INSTS_VALU, same runs).
 # gfx950 issues one wave64 VALU instruction per SIMD every 4 cycles:
 # 256 CUs x 4 SIMDs x 2.4 GHz / 4 = 6.144e11 wave-instructions/s.
-MEASURED_VALU_PER_EVENT = {"history": 1.799, "event": 3.459}
+MEASURED_VALU_PER_EVENT = {"history": 1.717, "event": 3.223}
 VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
 
 
@@ -510,7 +510,9 @@ def main():
                               "bound by instruction issue and latency (see valu; 4 waves "
                               "per SIMD, the LDS tally of 10,000 cells filling the CU)") \
                              if args.kernel == "history" else
-                             "72 algorithmic B/event; measured 88 B/event (44-byte pool slots)",
+                             ("72 algorithmic B/event; measured 55.7 B/event: a particle "
+                              "that crossed keeps x implicit, and passes between "
+                              "compactions write back only what the event changed"),
                      "valu": valu_rate(args.kernel, ev_step, k_avg_s)},
         "cpu_baseline": None,
         "variants": {},
