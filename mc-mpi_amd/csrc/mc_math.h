@@ -411,8 +411,13 @@ MC_FN double mc_exp_k(double x, const mc_coef *K) {
   /* Tiny |x| (zero and subnormals included) need no test of their own:
      kd = 0, r = x, and the fast path's error stays relative to exp(x) ~ 1
      (a failed rounding test goes to mc_exp_tiny below). */
-  const double kd = __builtin_rint(x * MC_INV_LN2_1024);
-  const int k = (int)kd;
+  /* k = the integer nearest x*1024/ln2, from one fma against 1.5*2^52: the
+     sum is rounded to an integer (|x*1024/ln2| < 2^20), whose value sits in
+     the low bits of the double; kd = k exactly.  (Any integer within 1/2 of
+     x*1024/ln2 serves: it only chooses the reduction.) */
+  const double shifted = mc_fma(x, MC_INV_LN2_1024, 0x1.8p52);
+  const double kd = shifted - 0x1.8p52;
+  const int k = (int)(uint32_t)mc_asu64(shifted);
   const int j = k & 1023;
   const double th = mc_exp_tab10[j][0], tl = mc_exp_tab10[j][1];
 #if defined(__HIP_DEVICE_COMPILE__)
