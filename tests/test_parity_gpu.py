@@ -354,10 +354,12 @@ def test_repeated_calls_accumulate(ctx, oracle):
 @pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("name,n", [("config3", 20_000_000), ("config2", 1_000_000)])
 def test_full_size_properties_and_sampled_histories(ctx, oracle, name, n, kernel):
-    """At BASELINE sizes: counts and weight balance close exactly, every exit
-    state is consistent, and a random sample of histories, re-run alone on
-    the oracle from their initial states, matches bit for bit (histories are
-    independent, so the sample is a valid per-history check at scale)."""
+    """At full bank sizes (2e7 inflow particles on the rank-2 layer of the
+    5,000-cell slab; 1e6 on the 10,000-cell slab): counts and weight balance
+    close exactly, every exit state is consistent, and a random sample of
+    histories, re-run alone on the oracle from their initial states, matches
+    bit for bit (histories are independent, so the sample is a valid
+    per-history check at scale)."""
     import torch
     import mcgpu
     wl = _workload(name, n)
