@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Benchmark: particle-events/s of the MI355X transport on one GPU.
 
-Headline (`value`): config 2 -- one slab of 10,000 cells on [0,1] with
+Headline (`value`): one slab of 10,000 cells on [0,1] with
 per-cell sigma_a ~ LogNormal(0, 0.5), sigma_s ~ LogNormal(0.5, 0.5), 1e8
 particles entering at x = 0 with mu = sqrt(U), weight 1, minimum weight 1e-3,
 FP64.  One step = one mcgpu_simulate_device call transporting the whole bank
@@ -9,19 +9,19 @@ FP64.  One step = one mcgpu_simulate_device call transporting the whole bank
 restore of the bank from a pristine device copy.
 
 Also reported, in "variants":
-  event     -- the event-based kernel on the same config-2 bank (one step);
+  event     -- the event-based kernel on the same bank (one step);
   host_api  -- mcgpu_simulate with a host bank (copies over PCIe included) on
-               the first 1e7 particles of config 2;
+               the first 1e7 particles of the 10,000-cell slab;
   cycle     -- the reference's headline job (its scripts' config.yaml: 1000
                cells, 1e8 particles, 1e5 per cycle, sync) with 1 and 8 ranks,
                every rank on this GPU (mcgpu_run --ranks R, device-resident
                cycle loop), next to the reference's published wall times;
-  batched   -- config 4 (4096 independent slabs, 5.8e8 particles) in one
+  batched   -- 4096 independent slabs (5.8e8 particles) in one
                persistent launch replayed from a hipGraph, launch shape chosen
                by the autotuner on a quarter of the slabs; and the same
                quarter of the slabs run one mcgpu_simulate_device call per
                slab ("per-problem launches") next to its batched time;
-  adjoint   -- config 5 (1,000 cells, 1e7 particles): forward with
+  adjoint   -- one slab of 1,000 cells, 1e7 particles: forward with
                per-history scores + seed-replay adjoint, against the plain
                forward, for d(absorbed weight in cells 800-899)/d(sigma);
   adjoint_batched -- the same across 64 such slabs (6.4e8 particles) with
@@ -97,7 +97,7 @@ def restore(dst, src):
 
 
 def cpu_baselines(n_ref, n_port):
-    """CPU legs on a prefix of the config-2 bank (n_ref resp. n_port
+    """CPU legs on a prefix of the 10,000-cell slab's bank (n_ref resp. n_port
     particles, each about 10 s of work), all host cores up to 16:
       reference -- the reference's own Layer::simulate (src/layer.cpp:220-361,
                    FP32 as the reference is built, OpenMP) from oracle/_ref;
@@ -108,7 +108,7 @@ def cpu_baselines(n_ref, n_port):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from mcgpu import configs
-    wl = configs.config2(max(n_ref, n_port))
+    wl = configs.slab10k(max(n_ref, n_port))
     threads = min(16, os.cpu_count() or 1)
 
     t0 = time.perf_counter()
@@ -118,7 +118,7 @@ def cpu_baselines(n_ref, n_port):
     dt = time.perf_counter() - t0
     ev = int(o["events"].sum())
     port = {"value": ev / dt, "unit": "particle-events/s", "cores": threads, "kind": "port",
-            "sample": f"first {n_port} particles of the config-2 bank ({ev} events, "
+            "sample": f"first {n_port} particles of the 10,000-cell slab's bank ({ev} events, "
                       f"{dt:.1f} s on {threads} threads; FP64 oracle restating "
                       "src/layer.cpp with correctly rounded log/exp)"}
     log(f"cpu port: {port['value']:.3e} events/s")
@@ -143,7 +143,7 @@ def cpu_baselines(n_ref, n_port):
     assert R.nb_active == 0
     ref = {"value": ev32 / dt, "unit": "particle-events/s", "cores": threads,
            "kind": "reference",
-           "sample": f"first {n_ref} particles of the config-2 bank in FP32 ({ev32} events, "
+           "sample": f"first {n_ref} particles of the 10,000-cell slab's bank in FP32 ({ev32} events, "
                      f"{dt:.1f} s): the reference's Layer::simulate(-1, {threads}) built from "
                      "src/layer.cpp + src/random.cpp (-O3 -fopenmp), one rank, no MPI"}
     log(f"cpu reference: {ref['value']:.3e} events/s")
@@ -177,9 +177,9 @@ def cpu_reference_job(n):
 
 def run_plain(ctx, args, torch, mcgpu, configs):
     n = args.particles
-    wl = configs.config2(n, with_bank=False)
-    log(f"config 2: building the {n}-particle bank on the device")
-    bank0 = device_bank(ctx, n, 2 * 1000 + 3, torch)  # config2_mu
+    wl = configs.slab10k(n, with_bank=False)
+    log(f"10,000-cell slab: building the {n}-particle bank on the device")
+    bank0 = device_bank(ctx, n, 2 * 1000 + 3, torch)  # slab10k_mu
     bank = {k: v.clone() for k, v in bank0.items()}
     sigs = torch.from_numpy(wl.sigs).cuda()
     ar = torch.from_numpy(wl.absorption_rates).cuda()
@@ -225,16 +225,16 @@ def run_plain(ctx, args, torch, mcgpu, configs):
 def run_host_api(ctx, args, torch, mcgpu, configs):
     """The drop-in path with host buffers (mcgpu_simulate: host AoS bank in,
     copied to the device and back inside the call), timed on the wall clock
-    around the call, on the first 1e7 particles of config 2."""
+    around the call, on the first 1e7 particles of the 10,000-cell slab."""
     n = args.host_particles
-    wl = configs.config2(n)
+    wl = configs.slab10k(n)
     bank = wl.bank.copy()
     w = np.zeros(wl.m)
     t0 = time.perf_counter()
     _, st, _ = ctx.simulate(bank, wl.sigs, wl.absorption_rates, w, wl.min_index, wl.max_index,
                             wl.dx, wl.min_weight)
     dt = time.perf_counter() - t0
-    out = {"workload": f"config2, first {n} particles, host AoS bank (40 B per particle)",
+    out = {"workload": f"single slab of 10000 cells, first {n} particles, host AoS bank (40 B per particle)",
            "events": st["events"], "wall_s": dt, "kernel_ms": st["kernel_ms"],
            "events_per_s_pcie_inclusive": st["events"] / dt,
            "events_per_s_kernel": st["events"] / st["kernel_ms"] * 1e3}
@@ -297,9 +297,9 @@ def run_cycle(ctx, args, torch, mcgpu, configs):
 
 
 def run_batched(ctx, args, torch, mcgpu, configs):
-    log("config 4: layout and bank")
-    slabs, sigs, ar, total = configs.config4_layout(particle_scale=args.c4_scale)
-    bank0 = device_bank(ctx, total, 4 * 1000 + 4, torch)  # config4_mu
+    log("4096-slab batch: layout and bank")
+    slabs, sigs, ar, total = configs.slabs4096_layout(particle_scale=args.batch_scale)
+    bank0 = device_bank(ctx, total, 4 * 1000 + 4, torch)  # slabs4096_mu
     bank = {k: v.clone() for k, v in bank0.items()}
     d_sigs = torch.from_numpy(sigs).cuda()
     d_ar = torch.from_numpy(ar).cuda()
@@ -343,7 +343,7 @@ def run_batched(ctx, args, torch, mcgpu, configs):
     st = plan.run()
     plan.close()
     log(f"full batch: {st['events']} events, {st['kernel_ms']:.0f} ms")
-    out = {"workload": f"config4: {len(slabs)} slabs, {total} particles, {len(sigs)} cells",
+    out = {"workload": f"{len(slabs)} independent slabs, {total} particles, {len(sigs)} cells",
            "events": st["events"], "graph_ms": st["kernel_ms"],
            "events_per_s": st["events"] / st["kernel_ms"] * 1e3, "autotuned": chosen,
            "autotune_ms": times,
@@ -359,7 +359,7 @@ def run_batched(ctx, args, torch, mcgpu, configs):
 
 
 def run_adjoint(ctx, args, torch, mcgpu, configs):
-    wl = configs.config5(args.c5_particles)
+    wl = configs.adjoint1000(args.adjoint_particles)
     init = mcgpu.bank_to_device(wl.bank)
     sa = torch.from_numpy(wl.sigma_a).cuda()
     ss = torch.from_numpy(wl.sigma_s).cuda()
@@ -380,7 +380,7 @@ def run_adjoint(ctx, args, torch, mcgpu, configs):
         adj = ctx.adjoint_cells(init, sa, ss, 0, wl.m, wl.dx, wl.min_weight, g, ga, gi,
                                 history_score=score)
         adj2 = ctx.adjoint_cells(init, sa, ss, 0, wl.m, wl.dx, wl.min_weight, g, ga, gi)
-        res = {"workload": f"config5: 1000 cells, {n} particles, target cells 800-899",
+        res = {"workload": f"single slab, 1000 cells, {n} particles, target cells 800-899",
                "forward_ms": plain["kernel_ms"], "forward_events": plain["events"],
                "forward_scored_ms": fwd["kernel_ms"], "adjoint_ms": adj["kernel_ms"],
                "adjoint_without_scores_ms": adj2["kernel_ms"],
@@ -393,11 +393,11 @@ def run_adjoint(ctx, args, torch, mcgpu, configs):
 
 
 def run_adjoint_batched(ctx, args, torch, mcgpu, configs):
-    """Config 5 batched across 64 slabs: batched forward with scores +
+    """The 1,000-cell adjoint workload across 64 slabs: batched forward with scores +
     batched adjoint against the batched plain forward."""
-    slabs, sa, ss, g, total = configs.config5_batch(64, args.c5b_particles)
-    log(f"config 5 batched: {len(slabs)} slabs, {total} particles")
-    init = device_bank(ctx, total, 5004, torch)  # config5_batch_mu
+    slabs, sa, ss, g, total = configs.adjoint1000_batch(64, args.adjoint_batch_particles)
+    log(f"1,000-cell slabs batched: {len(slabs)} slabs, {total} particles")
+    init = device_bank(ctx, total, 5004, torch)  # adjoint1000_batch_mu
     d_sa, d_ss, d_g = (torch.from_numpy(v).cuda() for v in (sa, ss, g))
     w = torch.zeros(len(sa), dtype=torch.float64, device="cuda")
     work = {k: v.clone() for k, v in init.items()}
@@ -411,7 +411,7 @@ def run_adjoint_batched(ctx, args, torch, mcgpu, configs):
     ga = torch.zeros(len(sa), dtype=torch.float64, device="cuda")
     gi = torch.zeros(len(sa), dtype=torch.float64, device="cuda")
     adj = ctx.adjoint_cells_batch(slabs, init, d_sa, d_ss, d_g, ga, gi, history_score=score)
-    out = {"workload": f"config5 x64: 64 slabs of 1000 cells, {total} particles, "
+    out = {"workload": f"64 slabs of 1000 cells, {total} particles, "
                        "target cells 800-899 of every slab",
            "forward_ms": plain["kernel_ms"], "forward_events": plain["events"],
            "forward_scored_ms": fwd["kernel_ms"], "adjoint_ms": adj["kernel_ms"],
@@ -441,13 +441,13 @@ def main():
                     help="particles for the reference CPU leg")
     ap.add_argument("--cpu-port-sample", type=int, default=200_000,
                     help="particles for the FP64 oracle CPU leg")
-    ap.add_argument("--c4-scale", type=float, default=1.0, help="config 4 particle scale")
-    ap.add_argument("--c5-particles", type=int, default=10_000_000)
-    ap.add_argument("--c5b-particles", type=int, default=10_000_000,
-                    help="particles per slab of the batched config 5")
+    ap.add_argument("--batch-scale", type=float, default=1.0, help="particle scale of the 4096-slab batch")
+    ap.add_argument("--adjoint-particles", type=int, default=10_000_000)
+    ap.add_argument("--adjoint-batch-particles", type=int, default=10_000_000,
+                    help="particles per slab of the batched 1,000-cell slabs")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-only", action="store_true",
-                    help="only the config-2 steps (for profiling runs)")
+                    help="only the 10,000-cell slab steps (for profiling runs)")
     ap.add_argument("--skip", default="",
                     help="comma-separated variants to leave out (host_api, cycle, batched, "
                          "adjoint, adjoint_batched)")
@@ -498,7 +498,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"config2: single slab, 10000 cells, {args.particles} particles "
+        "config": {"workload": f"single slab, 10000 cells, {args.particles} particles "
                                "from x=0, mu=sqrt(U), min weight 1e-3, FP64",
                    "kernel": args.kernel, "events_per_step": ev_step},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -63,7 +63,7 @@ struct pool_ref {
 };
 
 /* Pool accesses use ordinary loads and stores by default: measured on
-   MI355X (config 2, 2e7 particles) non-temporal hints made the kernel 40 %
+   MI355X (10,000-cell slab, 2e7 particles) non-temporal hints made the kernel 40 %
    slower, as the pools (about 180 MB at 8 slots per lane) then no longer stay
    in the Infinity Cache.  The hints remain selectable for experiments. */
 #ifndef MC_POOL_NT_LOAD
@@ -341,7 +341,7 @@ int launch_one(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_optio
     if (grid < 1) grid = 1;
   }
   /* compaction every 16th pass: the passes between write back only what
-     changed (measured on config 2: every pass 10.13 s, every 4th 9.25 s,
+     changed (measured on the 10,000-cell slab: every pass 10.13 s, every 4th 9.25 s,
      8th 9.11 s, 16th 9.04 s, 32nd 9.02 s) */
   const int c = o.compaction_interval > 0 ? o.compaction_interval : 16;
   const size_t waves = (size_t)grid * (size_t)(block / 64);

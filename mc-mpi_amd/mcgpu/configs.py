@@ -132,7 +132,7 @@ def _bank(n, x, index, mu, wmc, seed_start=0):
     return b
 
 
-def config1(n=100_000):
+def slab100(n=100_000):
     """1-D slab [0,1], 100 cells; sigma_a = 0.5 + 0.5 x_c, sigma_s = 1 - 0.5 x_c
     (so sigs = 1.5 and ar = sigma_a / 1.5); n particles at x = 0, mu = 1,
     weight 1; minimum weight 1e-2."""
@@ -142,35 +142,35 @@ def config1(n=100_000):
     sigs = np.full(m, 1.5)
     ar = (0.5 + 0.5 * xc) / 1.5
     bank = _bank(n, 0.0, 0, 1.0, 1.0)
-    return Workload("config1", 0.0, 1.0, 0, m, dx, 1e-2, sigs, ar, bank, True, True)
+    return Workload("slab100", 0.0, 1.0, 0, m, dx, 1e-2, sigs, ar, bank, True, True)
 
 
-def config2(n=100_000_000, seed=2, with_bank=True):
+def slab10k(n=100_000_000, seed=2, with_bank=True):
     """10 000 cells over [0,1]; sigma_a ~ LogNormal(0, 0.5), sigma_s ~
     LogNormal(0.5, 0.5) per cell; n particles at x = 0 with mu = sqrt(U),
     weight 1; minimum weight 1e-3.  with_bank=False skips the bank (the
-    benchmark builds it on the device, see config2_mu)."""
+    benchmark builds it on the device, see slab10k_mu)."""
     m = 10_000
     dx = 1.0 / m
     sa = lognormal_cells(seed * 1000 + 1, m, 0.0, 0.5)
     ss = lognormal_cells(seed * 1000 + 2, m, 0.5, 0.5)
     sigs = sa + ss
     ar = sa / sigs
-    bank = _bank(n, 0.0, 0, config2_mu(n, seed), 1.0) if with_bank else None
-    return Workload("config2", 0.0, 1.0, 0, m, dx, 1e-3, sigs, ar, bank, True, True)
+    bank = _bank(n, 0.0, 0, slab10k_mu(n, seed), 1.0) if with_bank else None
+    return Workload("slab10k", 0.0, 1.0, 0, m, dx, 1e-3, sigs, ar, bank, True, True)
 
 
-def config2_mu(n, seed=2, offset=0):
-    """Directions of config 2's source particles offset .. offset+n-1."""
+def slab10k_mu(n, seed=2, offset=0):
+    """Directions of the 10,000-cell slab's source particles offset .. offset+n-1."""
     return np.sqrt(uniform(seed * 1000 + 3, n, offset))
 
 
-def config3(n=20_000_000, seed=3):
+def layer_rank2of5(n=20_000_000, seed=3):
     """One rank's share of a 5-way decomposed 5000-cell slab: rank 2 owns
     cells [2000, 3000) = [0.4, 0.6).  Geometry from decompose_domain
     (src/layer.cpp:17-42) in FP64: dx_g = 1/5000, x_min = 2000 dx_g,
     x_max = 3000 dx_g, layer dx = (x_max - x_min)/1000.  Cross sections: the
-    config-1 law at the layer's cell centres x_min + i dx + dx/2.  Inflow: n/2
+    100-cell slab's law at the layer's cell centres x_min + i dx + dx/2.  Inflow: n/2
     particles at x_min (index 2000, mu = +sqrt(U)) and n/2 at x_max (index
     2999, mu = -sqrt(U)), weights U(0.1, 1); minimum weight 1e-2."""
     nb_cells, R, rank = 5000, 5, 2
@@ -191,10 +191,10 @@ def config3(n=20_000_000, seed=3):
     index = np.where(np.arange(n) < half, start, start + m - 1).astype(np.int32)
     mu = np.where(np.arange(n) < half, np.sqrt(u), -np.sqrt(u))
     bank = _bank(n, x, index, mu, w)
-    return Workload("config3", x_min, x_max, start, m, dx, 1e-2, sigs, ar, bank, False, False)
+    return Workload("layer_rank2of5", x_min, x_max, start, m, dx, 1e-2, sigs, ar, bank, False, False)
 
 
-def config5(n=10_000_000, seed=5):
+def adjoint1000(n=10_000_000, seed=5):
     """1000 cells over [0,1]; sigma_a, sigma_s ~ LogNormal(0, 0.5); n particles
     at x = 0 with mu = sqrt(U), weight 1; minimum weight 1e-3.  Target of the
     gradient: absorbed weight in cells 800-899."""
@@ -206,22 +206,22 @@ def config5(n=10_000_000, seed=5):
     ar = sa / sigs
     mu = np.sqrt(uniform(seed * 1000 + 3, n))
     bank = _bank(n, 0.0, 0, mu, 1.0)
-    wl = Workload("config5", 0.0, 1.0, 0, m, dx, 1e-3, sigs, ar, bank, True, True)
+    wl = Workload("adjoint1000", 0.0, 1.0, 0, m, dx, 1e-3, sigs, ar, bank, True, True)
     wl.sigma_a, wl.sigma_s = sa, ss
     wl.target = np.zeros(m)
     wl.target[800:900] = 1.0
     return wl
 
 
-def config5_batch(n_slabs=64, n_per_slab=10_000_000):
-    """Config 5 batched: n_slabs independent 1000-cell slabs on [0,1]; slab s
+def adjoint1000_batch(n_slabs=64, n_per_slab=10_000_000):
+    """The 1,000-cell adjoint slab batched: n_slabs independent 1000-cell slabs on [0,1]; slab s
     has its own LogNormal(0, 0.5) sigma_a, sigma_s (seeds 5000*s + 5001 and
-    5000*s + 5002, so slab 0 has config 5's cross sections) and n_per_slab
+    5000*s + 5002, so slab 0 has the 1,000-cell adjoint slab's cross sections) and n_per_slab
     particles at x = 0, weight 1, minimum weight 1e-3; the target is cells
     800-899 of every slab.  Particle i of the concatenated bank takes output
     i+1 of the reference seed chain and mu = sqrt(uniform(5004, .)[i]).
     Returns (slabs, sigma_a, sigma_s, g, total_particles) in the layout of
-    mcgpu_forward_cells_batch (slab tuples as in config4_layout)."""
+    mcgpu_forward_cells_batch (slab tuples as in slabs4096_layout)."""
     m = 1000
     slabs, sa, ss, g = [], [], [], []
     for s in range(n_slabs):
@@ -234,12 +234,12 @@ def config5_batch(n_slabs=64, n_per_slab=10_000_000):
     return slabs, np.concatenate(sa), np.concatenate(ss), np.concatenate(g), n_slabs * n_per_slab
 
 
-def config5_batch_mu(n, offset=0):
-    """Directions of the batched config 5's particles offset .. offset+n-1."""
+def adjoint1000_batch_mu(n, offset=0):
+    """Directions of the batched 1,000-cell slabs' particles offset .. offset+n-1."""
     return np.sqrt(uniform(5004, n, offset))
 
 
-def config4_layout(n_slabs=4096, seed=4, particle_scale=1.0):
+def slabs4096_layout(n_slabs=4096, seed=4, particle_scale=1.0):
     """4096 independent slabs on [0,1]: cells log-uniform in [32, 8192],
     particles log-uniform in [1e3, 1e6] (times particle_scale, at least 1),
     per cell sigma_t ~ LogNormal(0, 1) and scattering ratio c ~ U(0.1, 0.99)
@@ -267,12 +267,12 @@ def config4_layout(n_slabs=4096, seed=4, particle_scale=1.0):
     return slabs, st, 1.0 - c, poff
 
 
-def config4_mu(n, offset=0, seed=4):
-    """Directions of config 4's source particles offset .. offset+n-1."""
+def slabs4096_mu(n, offset=0, seed=4):
+    """Directions of the 4096-slab batch's source particles offset .. offset+n-1."""
     return np.sqrt(uniform(seed * 1000 + 4, n, offset))
 
 
-def config4_bank(total_particles, offset=0, count=None, seed=4):
-    """Host bank of config 4's particles offset .. offset+count-1."""
+def slabs4096_bank(total_particles, offset=0, count=None, seed=4):
+    """Host bank of the 4096-slab batch's particles offset .. offset+count-1."""
     count = total_particles - offset if count is None else count
-    return _bank(count, 0.0, 0, config4_mu(count, offset, seed), 1.0, seed_start=offset)
+    return _bank(count, 0.0, 0, slabs4096_mu(count, offset, seed), 1.0, seed_start=offset)

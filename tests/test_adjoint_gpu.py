@@ -19,7 +19,7 @@ def _say(msg):
 
 def _problem(n):
     from mcgpu import configs
-    wl = configs.config5(n)
+    wl = configs.adjoint1000(n)
     return wl, wl.sigma_a.copy(), wl.sigma_s.copy()
 
 

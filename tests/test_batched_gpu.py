@@ -19,7 +19,7 @@ def _say(msg):
 
 def _batch_inputs(n_slabs, scale, extra_slabs=()):
     from mcgpu import configs
-    slabs, sigs, ar, total = configs.config4_layout(n_slabs=n_slabs, particle_scale=scale)
+    slabs, sigs, ar, total = configs.slabs4096_layout(n_slabs=n_slabs, particle_scale=scale)
     slabs = list(slabs)
     sigs, ar = list(sigs), list(ar)
     coff = len(sigs)
@@ -30,7 +30,7 @@ def _batch_inputs(n_slabs, scale, extra_slabs=()):
         slabs.append((0, m, 1.0 / m, 1e-3, coff, total, n))
         coff += m
         total += n
-    bank = configs.config4_bank(total)
+    bank = configs.slabs4096_bank(total)
     return slabs, np.array(sigs), np.array(ar), bank
 
 

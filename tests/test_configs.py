@@ -21,10 +21,10 @@ def test_seed_chain_equals_oracle(oracle):
                           oracle.rnd_seed_chain(configs.ROOT_SEED, 5000)[4000:])
 
 
-def test_config5_batch_layout():
+def test_adjoint1000_batch_layout():
     from mcgpu import configs
-    slabs, sa, ss, g, total = configs.config5_batch(4, 1000)
-    w = configs.config5(10)
+    slabs, sa, ss, g, total = configs.adjoint1000_batch(4, 1000)
+    w = configs.adjoint1000(10)
     assert total == 4000 and len(sa) == 4000 and g.sum() == 400
     assert np.array_equal(sa[:1000], w.sigma_a) and np.array_equal(ss[:1000], w.sigma_s)
     assert [s[4] for s in slabs] == [0, 1000, 2000, 3000]

@@ -323,7 +323,7 @@ def test_kill_discards_remaining_weight(oracle):
 
 def test_history_results_do_not_depend_on_bank_order(oracle):
     from mcgpu import configs
-    wl = configs.config1(3000)
+    wl = configs.slab100(3000)
     perm = np.random.default_rng(1).permutation(3000)
     a = _run64(oracle, wl.m, wl.sigs, wl.absorption_rates, wl.bank, wl.min_weight)
     b = _run64(oracle, wl.m, wl.sigs, wl.absorption_rates, wl.bank[perm], wl.min_weight)

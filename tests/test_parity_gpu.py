@@ -24,8 +24,8 @@ def _say(msg):
 
 def _workload(name, n):
     from mcgpu import configs
-    return {"config1": configs.config1, "config2": configs.config2, "config3": configs.config3,
-            "config5": configs.config5}[name](n)
+    return {"slab100": configs.slab100, "slab10k": configs.slab10k, "layer_rank2of5": configs.layer_rank2of5,
+            "adjoint1000": configs.adjoint1000}[name](n)
 
 
 def _oracle_run(oracle, wl, bank=None, weights=None):
@@ -140,7 +140,7 @@ def test_device_create_particles(ctx, oracle):
 
 # ------------------------------------------------------------- transport
 
-CASES = [("config1", 20_000), ("config3", 20_000), ("config5", 3_000), ("config2", 400)]
+CASES = [("slab100", 20_000), ("layer_rank2of5", 20_000), ("adjoint1000", 3_000), ("slab10k", 400)]
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
@@ -204,7 +204,7 @@ LAUNCH_OPTS = [
 @pytest.mark.parametrize("opts", LAUNCH_OPTS)
 def test_launch_shape_does_not_change_results(ctx, oracle, opts):
     import mcgpu
-    wl = _workload("config1", 5_000)
+    wl = _workload("slab100", 5_000)
     _say(f"launch options {opts}")
     o = _oracle_run(oracle, wl)
     bank = wl.bank.copy()
@@ -302,7 +302,7 @@ def test_empty_bank(ctx):
 @pytest.mark.parametrize("kernel", KERNELS)
 def test_event_cap_reports_error(ctx, oracle, kernel):
     import mcgpu
-    wl = _workload("config1", 500)
+    wl = _workload("slab100", 500)
     b = wl.bank.copy()
     w = np.zeros(wl.m)
     result, st, rc = ctx.simulate(b, wl.sigs, wl.absorption_rates, w, wl.min_index, wl.max_index,
@@ -335,7 +335,7 @@ def test_out_of_range_tally_term(ctx, oracle):
 def test_repeated_calls_accumulate(ctx, oracle):
     """Two calls on two halves equal one call on the whole (per history), and
     the tally adds the two exact partial sums."""
-    wl = _workload("config5", 2000)
+    wl = _workload("adjoint1000", 2000)
     o1 = _oracle_run(oracle, wl, bank=wl.bank[:1000])
     o2 = _oracle_run(oracle, wl, bank=wl.bank[1000:])
     w = np.zeros(wl.m)
@@ -352,7 +352,7 @@ def test_repeated_calls_accumulate(ctx, oracle):
 # ---------------------------------------------------- full-size properties
 
 @pytest.mark.parametrize("kernel", KERNELS)
-@pytest.mark.parametrize("name,n", [("config3", 20_000_000), ("config2", 1_000_000)])
+@pytest.mark.parametrize("name,n", [("layer_rank2of5", 20_000_000), ("slab10k", 1_000_000)])
 def test_full_size_properties_and_sampled_histories(ctx, oracle, name, n, kernel):
     """At full bank sizes (2e7 inflow particles on the rank-2 layer of the
     5,000-cell slab; 1e6 on the 10,000-cell slab): counts and weight balance
