@@ -48,10 +48,11 @@ BYTES_PER_EVENT = 72  # algorithmic: the 36-byte particle state read + written p
 # this workload (1e7 particles), scaled by the copy-kernel calibration (read
 # x2.0, write x1.0); the counter runs are summarised under profiles/.
 MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0616, "event": 55.7}
-# VALU wave-instructions per event (rocprofv3 --pmc SQ_INSTS_VALU, same runs).
+# VALU wave-instructions per event (rocprofv3 --pmc SQ_INSTS_VALU, same
+# workload, on the current build).
 # gfx950 issues one wave64 VALU instruction per SIMD every 4 cycles:
 # 256 CUs x 4 SIMDs x 2.4 GHz / 4 = 6.144e11 wave-instructions/s.
-MEASURED_VALU_PER_EVENT = {"history": 1.717, "event": 3.223}
+MEASURED_VALU_PER_EVENT = {"history": 1.701, "event": 3.205}
 VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
 
 
