@@ -73,9 +73,18 @@ __global__ __launch_bounds__(1024) void k_adjoint(adj_args a, int refill_k) {
   double T = 0.0, P = 0.0;
   unsigned long long tot_ev = 0;
 
+  /* events between refills: refill_k if positive, else adapted per wave as
+     in the history kernel (halved when many lanes were idle at a refill,
+     doubled when almost none were) */
+  const bool adapt = refill_k <= 0;
+  int kk = adapt ? 128 : refill_k;
   for (;;) {
     if (!done) {
       const unsigned long long need = __ballot(!has);
+      if (adapt) {
+        const int idle = __popcll(need);
+        kk = idle > 16 ? (kk > 8 ? kk >> 1 : 8) : (idle < 4 ? (kk < 256 ? kk << 1 : 256) : kk);
+      }
       if (need) {
         const int cnt = __popcll(need);
         const int leader = __ffsll((long long)need) - 1;
@@ -109,7 +118,7 @@ __global__ __launch_bounds__(1024) void k_adjoint(adj_args a, int refill_k) {
     }
     if (__ballot(has) == 0) break;
     if (has) {
-      for (int k = 0; k < refill_k; ++k) {
+      for (int k = 0; k < kk; ++k) {
         if (!mc_alive(p, a.L) || ev >= max_ev) break;
         const int c = p.index - a.L.index_start;
         mc_cell cell;
@@ -259,7 +268,7 @@ int launch_adj(mcgpu_context *ctx, const adj_args &a, const mcgpu_options &o, hi
     const int64_t blocks_needed = (a.n + block - 1) / block;
     if (blocks_needed < grid) grid = (int)(blocks_needed > 0 ? blocks_needed : 1);
   }
-  const int k = o.events_per_refill > 0 ? o.events_per_refill : 8;
+  const int k = o.events_per_refill > 0 ? o.events_per_refill : 0; /* 0: adaptive */
   hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, s, a, k);
   MC_HIP(ctx, hipGetLastError());
   return MCGPU_OK;

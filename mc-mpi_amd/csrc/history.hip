@@ -64,9 +64,19 @@ __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refil
   double score = 0.0;
   unsigned long long tot_ev = 0, tot_col = 0, nl = 0, nr = 0, nk = 0;
 
+  /* events between refills: refill_k if positive, else adapted per wave
+     from the lanes found idle at each refill (many idle: the histories are
+     short against the interval, halve it; almost none: double it, since
+     each refill costs a queue atomic and a pass through this loop) */
+  const bool adapt = refill_k <= 0;
+  int kk = adapt ? 128 : refill_k;
   for (;;) {
     if (!done) { /* wave-uniform */
       const unsigned long long need = __ballot(!has);
+      if (adapt) {
+        const int idle = __popcll(need);
+        kk = idle > 16 ? (kk > 8 ? kk >> 1 : 8) : (idle < 4 ? (kk < 256 ? kk << 1 : 256) : kk);
+      }
       if (need) {
         const int cnt = __popcll(need);
         const int leader = __ffsll((long long)need) - 1;
@@ -90,7 +100,7 @@ __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refil
     }
     if (__ballot(has) == 0) break; /* wave-uniform */
     if (has) {
-      for (int k = 0; k < refill_k; ++k) {
+      for (int k = 0; k < kk; ++k) {
         if (!mc_alive(p, a.L) || ev >= a.max_events) break;
         const int c = p.index - a.L.index_start;
         bool coll;
@@ -167,7 +177,7 @@ int launch_one(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_optio
     int64_t blocks_needed = (waves * 64 + block - 1) / block;
     if (blocks_needed < grid) grid = (int)(blocks_needed > 0 ? blocks_needed : 1);
   }
-  int k = o.events_per_refill > 0 ? o.events_per_refill : 8;
+  int k = o.events_per_refill > 0 ? o.events_per_refill : 0; /* 0: adaptive */
   hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, s, a, k);
   MC_HIP(ctx, hipGetLastError());
   return MCGPU_OK;
