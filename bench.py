@@ -52,7 +52,7 @@ MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0616, "event": 55.7}
 # workload, on the current build).
 # gfx950 issues one wave64 VALU instruction per SIMD every 4 cycles:
 # 256 CUs x 4 SIMDs x 2.4 GHz / 4 = 6.144e11 wave-instructions/s.
-MEASURED_VALU_PER_EVENT = {"history": 1.701, "event": 3.205}
+MEASURED_VALU_PER_EVENT = {"history": 1.668, "event": 3.205}
 VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
 
 
