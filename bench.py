@@ -44,15 +44,18 @@ sys.path.insert(0, os.path.join(ROOT, "mc-mpi_amd"))
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_EVENT = 72  # algorithmic: the 36-byte particle state read + written per event
 
-# HBM bytes per event measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
-# this workload (1e7 particles), scaled by the copy-kernel calibration (read
-# x2.0, write x1.0); the counter runs are summarised under profiles/.
+# Per-event figures of the 10,000-cell slab measured with rocprofv3 on the
+# build this file ships with (1e7 particles; the summaries are
+# profiles/history_slab10k_counters.md and profiles/event_slab10k_hbm_counters.md):
+#  - bytes between L2 and the memory fabric (FETCH_SIZE / WRITE_SIZE), scaled by
+#    the copy-kernel calibration (read x2.0, write x1.0).  These are L2 misses:
+#    an upper bound on HBM bytes, since the Infinity Cache (MALL) in front of
+#    HBM can serve them;
+#  - VALU wave-instructions (SQ_INSTS_VALU).
 MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0616, "event": 55.7}
-# VALU wave-instructions per event (rocprofv3 --pmc SQ_INSTS_VALU, same
-# workload, on the current build).
+MEASURED_VALU_PER_EVENT = {"history": 1.668, "event": 3.205}
 # gfx950 issues one wave64 VALU instruction per SIMD every 4 cycles:
 # 256 CUs x 4 SIMDs x 2.4 GHz / 4 = 6.144e11 wave-instructions/s.
-MEASURED_VALU_PER_EVENT = {"history": 1.668, "event": 3.205}
 VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
 
 
@@ -63,6 +66,64 @@ def valu_rate(kernel, events, seconds):
     a = w * events / seconds
     return {"wave_instructions_per_event": w, "achieved": a, "peak": VALU_PEAK_WAVE_INSTR,
             "unit": "wave-instructions/s", "frac": a / VALU_PEAK_WAVE_INSTR}
+
+
+def roofline(kernel, events, seconds):
+    """The roofline block of one plain-kernel launch (events over seconds).
+
+    history kernel: the state stays in registers (about 0.06 B/event of
+      traffic), so HBM does not bound it; it is bound by VALU issue.  `frac`
+      is the issue rate (measured wave-instructions per event x events/s)
+      over gfx950's 6.144e11 wave-instructions/s; the bytes are a side field.
+    event kernel: bound by memory.  `frac` is the measured bytes per event
+      (L2-miss bytes, FETCH_SIZE + WRITE_SIZE) x events/s over 8 TB/s.  Its
+      ping-pong pools (about 180 MB) are sized to stay in the 256 MB MALL,
+      so those bytes are not all HBM bytes; the 72 algorithmic B/event
+      yardstick of the event formulation is reported beside it."""
+    measured_b = MEASURED_TRAFFIC_PER_EVENT.get(kernel)
+    traffic = None if measured_b is None else measured_b * events
+    alg_gbs = events * BYTES_PER_EVENT / seconds / 1e9
+    hbm = {"algorithmic_bytes_per_event": BYTES_PER_EVENT,
+           "algorithmic_GBs": alg_gbs, "algorithmic_frac": alg_gbs / HBM_PEAK_GBS,
+           "measured_bytes_per_event": measured_b,
+           "measured_GBs": None if traffic is None else traffic / seconds / 1e9,
+           "measured_frac": None if traffic is None else traffic / seconds / 1e9 / HBM_PEAK_GBS,
+           "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    valu = valu_rate(kernel, events, seconds)
+    if kernel == "history":
+        return {"bound": "valu", "achieved": valu["achieved"], "peak": valu["peak"],
+                "unit": valu["unit"], "frac": valu["frac"], "traffic": traffic,
+                "wave_instructions_per_event": valu["wave_instructions_per_event"],
+                "hbm": hbm,
+                "note": "history kernel: one history per lane, state in registers; "
+                        "bound by FP64 VALU issue (frac = measured wave-instructions/event x "
+                        "events/s / 6.144e11); traffic = measured L2-miss bytes per launch"}
+    return {"bound": "hbm", "achieved": hbm["measured_GBs"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": hbm["measured_frac"], "traffic": traffic,
+            "algorithmic": {"bytes_per_event": BYTES_PER_EVENT, "achieved": alg_gbs,
+                            "frac": alg_gbs / HBM_PEAK_GBS},
+            "valu": valu,
+            "note": "event kernel: achieved = measured L2-miss bytes (FETCH_SIZE + WRITE_SIZE, "
+                    "copy-calibrated) per event x events/s; the per-block pools (about "
+                    "180 MB) are sized to stay in the 256 MB MALL, so these bytes are an "
+                    "upper bound on HBM bytes"}
+
+
+def host_cores():
+    """(threads to use, nproc, CPU quota of this cgroup or None): all CPUs this
+    process may run on (os.sched_getaffinity, what nproc prints), capped by
+    the cgroup's CPU quota when one is set below that."""
+    nproc = len(os.sched_getaffinity(0))
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            q, period = f.read().split()
+        if q != "max":
+            quota = max(1, int(int(q) // int(period)))
+    except (OSError, ValueError):
+        pass
+    threads = nproc if quota is None else min(nproc, quota)
+    return threads, nproc, quota
 
 
 def log(msg):
@@ -97,20 +158,23 @@ def restore(dst, src):
         dst[k].copy_(src[k])
 
 
-def cpu_baselines(n_ref, n_port):
-    """CPU legs on a prefix of the 10,000-cell slab's bank (n_ref resp. n_port
-    particles, each about 10 s of work), all host cores up to 16:
+def cpu_baselines(n_ref, n_port, n_ref1):
+    """CPU legs on prefixes of the 10,000-cell slab's bank, each about 10 s
+    of work, on all host cores this process may use (host_cores()):
       reference -- the reference's own Layer::simulate (src/layer.cpp:220-361,
-                   FP32 as the reference is built, OpenMP) from oracle/_ref;
-                   events counted by the FP32 oracle on the same bank (its
-                   per-history results equal the reference's);
-      port      -- the FP64 oracle (the same arithmetic as the GPU path).
+                   FP32 as the reference is built, OpenMP) from oracle/_ref on
+                   the first n_ref particles, and on one thread on the first
+                   n_ref1; events counted by the FP32 oracle on the same bank
+                   (its per-history results equal the reference's);
+      port      -- the FP64 oracle (the same arithmetic as the GPU path) on the
+                   first n_port particles.
     Returns (reference or None when oracle/_ref is not built, port)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from mcgpu import configs
-    wl = configs.slab10k(max(n_ref, n_port))
-    threads = min(16, os.cpu_count() or 1)
+    wl = configs.slab10k(max(n_ref, n_port, n_ref1))
+    threads, nproc, quota = host_cores()
+    hostinfo = {"nproc": nproc, "cgroup_cpu_quota": quota}
 
     t0 = time.perf_counter()
     o = pyoracle.simulate_bank(64, wl.index_start, wl.m, wl.dx, wl.min_weight, wl.sigs,
@@ -119,6 +183,7 @@ def cpu_baselines(n_ref, n_port):
     dt = time.perf_counter() - t0
     ev = int(o["events"].sum())
     port = {"value": ev / dt, "unit": "particle-events/s", "cores": threads, "kind": "port",
+            **hostinfo,
             "sample": f"first {n_port} particles of the 10,000-cell slab's bank ({ev} events, "
                       f"{dt:.1f} s on {threads} threads; FP64 oracle restating "
                       "src/layer.cpp with correctly rounded log/exp)"}
@@ -126,28 +191,38 @@ def cpu_baselines(n_ref, n_port):
     if not os.path.exists(pyoracle.REF_PATH):
         return None, port
 
-    b32 = np.zeros(n_ref, dtype=pyoracle.P32)
-    for f in ("seed", "x", "mu", "wmc", "index"):
-        b32[f] = wl.bank[f][:n_ref]
     sigs32 = wl.sigs.astype(np.float32)
     ar32 = wl.absorption_rates.astype(np.float32)
-    R = pyoracle.RefLayer.new(0.0, 1.0, 0, wl.m, wl.min_weight)
-    R.sigs[:] = sigs32
-    R.absorption_rates[:] = ar32
-    o32 = pyoracle.simulate_bank(32, 0, wl.m, np.float32(R.dx), np.float32(wl.min_weight), sigs32,
-                                 ar32, b32, nthread=threads)
-    ev32 = int(o32["events"].sum())
-    R.push(b32)
-    t0 = time.perf_counter()
-    R.simulate(-1, threads)
-    dt = time.perf_counter() - t0
-    assert R.nb_active == 0
+
+    def ref_leg(n, nthread):
+        b32 = np.zeros(n, dtype=pyoracle.P32)
+        for f in ("seed", "x", "mu", "wmc", "index"):
+            b32[f] = wl.bank[f][:n]
+        R = pyoracle.RefLayer.new(0.0, 1.0, 0, wl.m, wl.min_weight)
+        R.sigs[:] = sigs32
+        R.absorption_rates[:] = ar32
+        o32 = pyoracle.simulate_bank(32, 0, wl.m, np.float32(R.dx), np.float32(wl.min_weight),
+                                     sigs32, ar32, b32, nthread=threads)
+        ev32 = int(o32["events"].sum())
+        R.push(b32)
+        t0 = time.perf_counter()
+        R.simulate(-1, nthread)
+        dt = time.perf_counter() - t0
+        assert R.nb_active == 0
+        return ev32, dt
+
+    ev32, dt = ref_leg(n_ref, threads)
+    ev1, dt1 = ref_leg(n_ref1, 1)
     ref = {"value": ev32 / dt, "unit": "particle-events/s", "cores": threads,
-           "kind": "reference",
-           "sample": f"first {n_ref} particles of the 10,000-cell slab's bank in FP32 ({ev32} events, "
-                     f"{dt:.1f} s): the reference's Layer::simulate(-1, {threads}) built from "
-                     "src/layer.cpp + src/random.cpp (-O3 -fopenmp), one rank, no MPI"}
-    log(f"cpu reference: {ref['value']:.3e} events/s")
+           "kind": "reference", **hostinfo,
+           "value_1_thread": ev1 / dt1,
+           "sample": f"first {n_ref} particles of the 10,000-cell slab's bank in FP32 ({ev32} "
+                     f"events, {dt:.1f} s): the reference's Layer::simulate(-1, {threads}) "
+                     "built from src/layer.cpp + src/random.cpp (-O3 -fopenmp), one rank, no "
+                     f"MPI; value_1_thread: the first {n_ref1} particles ({ev1} events, "
+                     f"{dt1:.1f} s) with Layer::simulate(-1, 1)"}
+    log(f"cpu reference: {ref['value']:.3e} events/s on {threads} threads, "
+        f"{ref['value_1_thread']:.3e} on 1 (nproc {nproc}, quota {quota})")
     return ref, port
 
 
@@ -161,7 +236,7 @@ def cpu_reference_job(n):
     import pyoracle
     if not os.path.exists(pyoracle.REF_PATH):
         return None
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_cores()[0]
     R = pyoracle.RefLayer.decompose_domain(0.0, 1.0, float(np.float32(0.7071067690849304)), 1, 0,
                                            1000, n, float(np.float32(1e-12)))
     t0 = time.perf_counter()
@@ -212,12 +287,50 @@ def run_plain(ctx, args, torch, mcgpu, configs):
            "value": sum(events) / elapsed}
     other = None
     if not args.plain_only:
+        # Both kernels on the same 1e8-particle bank, outside the timed region,
+        # with per-history exit codes and event / collision counts; every
+        # history's final state and counts and the tally must be bit-identical.
         other_kernel = "event" if args.kernel == "history" else "history"
+        outs = {}
+        for kern in (args.kernel, other_kernel):
+            res8 = torch.empty(n, dtype=torch.int8, device="cuda")
+            evs = torch.empty(n, dtype=torch.int64, device="cuda")
+            cols = torch.empty(n, dtype=torch.int64, device="cuda")
+            restore(bank, bank0)
+            w.zero_()
+            st, _ = ctx.simulate_device(bank, sigs, ar, w, wl.min_index, wl.max_index, wl.dx,
+                                        wl.min_weight, result=res8, events=evs, collisions=cols,
+                                        options=mcgpu.make_options(kern))
+            torch.cuda.synchronize()
+            log(f"{kern} kernel: {st['events']} events, kernel {st['kernel_ms']:.1f} ms")
+            outs[kern] = (st, {k: v.clone() for k, v in bank.items()}, w.clone(), res8, evs, cols)
+        a, b = outs[args.kernel], outs[other_kernel]
+        mism = {f: int((a[1][f].view(torch.int64 if a[1][f].element_size() == 8 else torch.int32)
+                        != b[1][f].view(torch.int64 if b[1][f].element_size() == 8 else torch.int32)).sum())
+                for f in bank}
+        mism["exit_code"] = int((a[3] != b[3]).sum())
+        mism["events"] = int((a[4] != b[4]).sum())
+        mism["collisions"] = int((a[5] != b[5]).sum())
+        tally_cells = int((a[2].view(torch.int64) != b[2].view(torch.int64)).sum())
+        same = all(v == 0 for v in mism.values()) and tally_cells == 0 and \
+            a[0]["events"] == b[0]["events"] and a[0]["collisions"] == b[0]["collisions"]
+        parity = {"kernels": [args.kernel, other_kernel], "histories": n,
+                  "bit_identical": same, "mismatching_histories_per_field": mism,
+                  "mismatching_tally_cells": tally_cells,
+                  "events": [a[0]["events"], b[0]["events"]],
+                  "collisions": [a[0]["collisions"], b[0]["collisions"]]}
+        log(f"history vs event kernel on the full bank: {parity}")
+        if not same:
+            log("ERROR: the two kernels disagree on the full bank")
+        del outs, a, b
+        # the other kernel timed on its own, without the per-history outputs
         st = step(mcgpu.make_options(other_kernel))
         torch.cuda.synchronize()
+        log(f"{other_kernel} kernel (timed): {st['events']} events, "
+            f"kernel {st['kernel_ms']:.1f} ms")
         other = {"kernel": other_kernel, "events": st["events"], "kernel_ms": st["kernel_ms"],
-                 "events_per_s": st["events"] / st["kernel_ms"] * 1e3}
-        log(f"{other_kernel} kernel: {st['events']} events, kernel {st['kernel_ms']:.1f} ms")
+                 "events_per_s": st["events"] / st["kernel_ms"] * 1e3,
+                 "full_bank_parity": parity}
     del bank, bank0
     torch.cuda.empty_cache()
     return res, other
@@ -316,25 +429,45 @@ def run_batched(ctx, args, torch, mcgpu, configs):
                              particles_per_lane=chosen["particles_per_lane"],
                              events_per_refill=chosen["events_per_refill"])
 
-    # batched vs per-problem launches on the same quarter
+    # batched vs per-problem launches on the same quarter; every slab's bank,
+    # exit codes and tally compared bit for bit between the two
     restore(bank, bank0)
     w.zero_()
-    plan = mcgpu.Batch(ctx, sub, d_sigs, d_ar, w, bank, options=opt)
+    res_b = torch.zeros(total, dtype=torch.int8, device="cuda")
+    plan = mcgpu.Batch(ctx, sub, d_sigs, d_ar, w, bank, result=res_b, options=opt)
     st_sub = plan.run()
     plan.close()
+    torch.cuda.synchronize()
+    out_b = ({k: v.clone() for k, v in bank.items()}, w.clone())
     restore(bank, bank0)
     w.zero_()
+    res_p = torch.zeros(total, dtype=torch.int8, device="cuda")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     ev_pp = 0
     for (lo, hi, dx, mw, coff, poff, n) in sub:
         b = {k: v[poff:poff + n] for k, v in bank.items()}
         st, _ = ctx.simulate_device(b, d_sigs[coff:coff + hi - lo], d_ar[coff:coff + hi - lo],
-                                    w[coff:coff + hi - lo], lo, hi, dx, mw)
+                                    w[coff:coff + hi - lo], lo, hi, dx, mw,
+                                    result=res_p[poff:poff + n])
         ev_pp += st["events"]
     torch.cuda.synchronize()
     t_pp = time.perf_counter() - t0
     log(f"quarter: batched {st_sub['kernel_ms']:.0f} ms, per-problem {t_pp * 1e3:.0f} ms")
+    mism = {f: int((out_b[0][f].view(torch.int64 if bank[f].element_size() == 8 else torch.int32)
+                    != bank[f].view(torch.int64 if bank[f].element_size() == 8 else torch.int32)).sum())
+            for f in bank}
+    mism["exit_code"] = int((res_b != res_p).sum())
+    tally_cells = int((out_b[1].view(torch.int64) != w.view(torch.int64)).sum())
+    n_sub = sum(s_[6] for s_ in sub)
+    same = all(v == 0 for v in mism.values()) and tally_cells == 0 and ev_pp == st_sub["events"]
+    quarter_parity = {"slabs": len(sub), "histories": n_sub, "bit_identical": same,
+                      "mismatching_histories_per_field": mism,
+                      "mismatching_tally_cells": tally_cells}
+    log(f"quarter, batched vs per-problem: {quarter_parity}")
+    if not same:
+        log("ERROR: batched and per-problem runs disagree")
+    del out_b, res_b, res_p
 
     # the full batch
     restore(bank, bank0)
@@ -353,7 +486,8 @@ def run_batched(ctx, args, torch, mcgpu, configs):
                        "per_problem_events": ev_pp,
                        "batched_events_per_s": st_sub["events"] / st_sub["kernel_ms"] * 1e3,
                        "per_problem_events_per_s": ev_pp / t_pp,
-                       "speedup": (st_sub["events"] / st_sub["kernel_ms"] * 1e3) / (ev_pp / t_pp)}}
+                       "speedup": (st_sub["events"] / st_sub["kernel_ms"] * 1e3) / (ev_pp / t_pp),
+                       "parity": quarter_parity}}
     del bank, bank0
     torch.cuda.empty_cache()
     return out
@@ -440,6 +574,8 @@ def main():
     ap.add_argument("--refill", type=int, default=0, help="history kernel: events per refill")
     ap.add_argument("--cpu-ref-sample", type=int, default=2_000_000,
                     help="particles for the reference CPU leg")
+    ap.add_argument("--cpu-ref1-sample", type=int, default=100_000,
+                    help="particles for the one-thread reference CPU leg")
     ap.add_argument("--cpu-port-sample", type=int, default=200_000,
                     help="particles for the FP64 oracle CPU leg")
     ap.add_argument("--batch-scale", type=float, default=1.0, help="particle scale of the 4096-slab batch")
@@ -484,8 +620,6 @@ def main():
     plain, other = run_plain(ctx, args, torch, mcgpu, configs)
     ev_step = plain["events_per_step"]
     k_avg_s = sum(plain["kernel_ms"]) / len(plain["kernel_ms"]) / 1e3
-    achieved = ev_step * BYTES_PER_EVENT / k_avg_s / 1e9
-    traffic = MEASURED_TRAFFIC_PER_EVENT.get(args.kernel)
     out = {
         "metric": "particle-events/s on 1 MI355X (plain vs batched vs fwd+adjoint); % HBM roofline",
         "value": plain["value"],
@@ -501,31 +635,14 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"single slab, 10000 cells, {args.particles} particles "
                                "from x=0, mu=sqrt(U), min weight 1e-3, FP64",
-                   "kernel": args.kernel, "events_per_step": ev_step},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None if traffic is None else traffic * ev_step,
-                     "note": ("72 algorithmic B/event (36-byte state read + written, the "
-                              "event formulation); the history kernel keeps the state in "
-                              "registers (measured 0.06 B/event), hence frac > 1; it is "
-                              "bound by instruction issue and latency (see valu; 4 waves "
-                              "per SIMD, the LDS tally of 10,000 cells filling the CU)") \
-                             if args.kernel == "history" else
-                             ("72 algorithmic B/event; measured 55.7 B/event: a particle "
-                              "that crossed keeps x implicit, and passes between "
-                              "compactions write back only what the event changed"),
-                     "valu": valu_rate(args.kernel, ev_step, k_avg_s)},
+                   "kernel": args.kernel, "events_per_step": ev_step,
+                   "kernel_ms_avg": k_avg_s * 1e3},
+        "roofline": roofline(args.kernel, ev_step, k_avg_s),
         "cpu_baseline": None,
         "variants": {},
     }
     if other is not None:
-        ev_o, t_o = other["events"], other["kernel_ms"] / 1e3
-        ach_o = ev_o * BYTES_PER_EVENT / t_o / 1e9
-        tr_o = MEASURED_TRAFFIC_PER_EVENT.get(other["kernel"])
-        other["roofline"] = {"bound": "hbm", "achieved": ach_o, "peak": HBM_PEAK_GBS,
-                             "unit": "GB/s", "frac": ach_o / HBM_PEAK_GBS,
-                             "traffic": None if tr_o is None else tr_o * ev_o,
-                             "valu": valu_rate(other["kernel"], ev_o, t_o)}
+        other["roofline"] = roofline(other["kernel"], other["events"], other["kernel_ms"] / 1e3)
         out["variants"][other["kernel"]] = other
     if not args.plain_only:
         runs = {"host_api": run_host_api, "cycle": run_cycle, "batched": run_batched,
@@ -536,7 +653,8 @@ def main():
             out["variants"][name] = fn(ctx, args, torch, mcgpu, configs)
     if not args.no_cpu:
         log("timing the CPU baselines")
-        ref, port = cpu_baselines(args.cpu_ref_sample, args.cpu_port_sample)
+        ref, port = cpu_baselines(args.cpu_ref_sample, args.cpu_port_sample,
+                                  args.cpu_ref1_sample)
         out["cpu_baseline"] = ref if ref is not None else port
         out["variants"]["cpu_fp64_port"] = port
         if "cycle" in out["variants"]:

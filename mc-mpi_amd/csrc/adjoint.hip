@@ -255,15 +255,18 @@ int launch_adj(mcgpu_context *ctx, const adj_args &a, const mcgpu_options &o, hi
   if (lds > 65536)
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  int block = o.block_size > 0 ? o.block_size : (lds > 81920 ? 1024 : 256);
-  if (block > 1024 || block % 64)
+  int per_cu = 0;
+  int block = o.block_size;
+  if (block <= 0) {
+    block = mc_auto_block(kern, lds, &per_cu);
+  } else if (block > 1024 || block % 64) {
     return mc_fail(ctx, MCGPU_ERR_ARG, "block_size must be a multiple of 64 and <= 1024");
+  } else if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, lds) !=
+                 hipSuccess || per_cu < 1) {
+    per_cu = 1;
+  }
   int grid = o.grid_blocks;
   if (grid <= 0) {
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, block, lds) != hipSuccess ||
-        per_cu < 1)
-      per_cu = 1;
     grid = per_cu * ctx->num_cus;
     const int64_t blocks_needed = (a.n + block - 1) / block;
     if (blocks_needed < grid) grid = (int)(blocks_needed > 0 ? blocks_needed : 1);

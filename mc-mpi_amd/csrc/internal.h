@@ -114,6 +114,33 @@ static __device__ __forceinline__ void store_state(const mc_transport_args &a, i
   }
 }
 
+/* Threads per block of a persistent kernel with `lds` bytes of dynamic LDS
+   per block, when the caller gave none: of 256, 512 and 1024, the one with
+   the most resident waves per CU by the occupancy calculator (which counts
+   registers and LDS); ties go to the smaller block.  With LDS-resident
+   tables a small block can leave most of the CU idle: 56 KB per block fits
+   two blocks in 160 KB, 8 waves at 256 threads but 16 at 512.
+   *per_cu receives the resident blocks per CU of the choice (at least 1). */
+template <typename K>
+static inline int mc_auto_block(K kern, size_t lds, int *per_cu) {
+  int best = 256, best_waves = -1, best_per_cu = 1;
+  for (int b = 256; b <= 1024; b *= 2) {
+    int pc = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc, kern, b, lds) != hipSuccess) {
+      (void)hipGetLastError();
+      pc = 0;
+    }
+    const int waves = pc * (b / 64);
+    if (waves > best_waves) {
+      best = b;
+      best_waves = waves;
+      best_per_cu = pc > 0 ? pc : 1;
+    }
+  }
+  *per_cu = best_per_cu;
+  return best;
+}
+
 int mc_launch_history(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_options &o,
                       hipStream_t s);
 int mc_launch_event(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_options &o,

@@ -13,7 +13,8 @@ from ._lib import (ERR_ARG, ERR_EVENT_CAP, ERR_HIP, ERR_NO_DEVICE, ERR_NOMEM, ER
                    McgpuError, Options, Slab, Stats, lib, status_string)
 
 __all__ = ["Context", "Batch", "PARTICLE", "McgpuError", "KERNEL_HISTORY", "KERNEL_EVENT", "lib",
-           "bank_to_device", "bank_from_device", "make_options", "options_dict"]
+           "bank_to_device", "bank_from_device", "make_options", "options_dict", "check_tensor",
+           "check_bank"]
 
 _KERNELS = {"history": KERNEL_HISTORY, "event": KERNEL_EVENT}
 
@@ -57,6 +58,71 @@ def _ptr(t):
     if isinstance(t, np.ndarray):
         return t.ctypes.data
     return t.data_ptr()
+
+
+_BANK_DTYPES = None
+
+
+def _bank_dtypes():
+    global _BANK_DTYPES
+    if _BANK_DTYPES is None:
+        import torch
+        _BANK_DTYPES = {"seed": (torch.int64, torch.uint64), "x": (torch.float64,),
+                        "mu": (torch.float64,), "wmc": (torch.float64,),
+                        "index": (torch.int32,)}
+    return _BANK_DTYPES
+
+
+def check_tensor(t, name, dtypes, n=None, device=None, optional=False):
+    """Validate a tensor handed to the C ABI, before any launch: a torch
+    tensor of one of `dtypes`, contiguous, with n elements (if given), in
+    device memory of HIP device `device` (if given).  The library reads and
+    writes exactly the element counts it is told, so a wrong dtype, length or
+    a host tensor would be out-of-bounds device memory access.  Raises
+    TypeError (not a tensor, wrong dtype) or ValueError (length, layout,
+    device)."""
+    import torch
+    if t is None:
+        if optional:
+            return
+        raise ValueError(f"{name}: required")
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{name}: expected a torch.Tensor, got {type(t).__name__}")
+    if not isinstance(dtypes, tuple):
+        dtypes = (dtypes,)
+    if t.dtype not in dtypes:
+        raise TypeError(f"{name}: dtype {t.dtype}, expected "
+                        + " or ".join(str(d) for d in dtypes))
+    if n is not None and t.numel() != n:
+        raise ValueError(f"{name}: {t.numel()} elements, expected {n}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: must be contiguous")
+    if device is not None and (t.device.type != "cuda" or t.device.index != device):
+        raise ValueError(f"{name}: on {t.device}, expected cuda:{device}")
+
+
+def check_bank(bank, name, n=None, device=None):
+    """Validate a device SoA bank (dict seed/x/mu/wmc/index); returns its length."""
+    if not isinstance(bank, dict) or set(bank) != set(_bank_dtypes()):
+        raise TypeError(f"{name}: expected a dict with keys seed, x, mu, wmc, index")
+    if n is None:
+        n = bank["x"].numel() if hasattr(bank["x"], "numel") else None
+    for f, dts in _bank_dtypes().items():
+        check_tensor(bank[f], f"{name}[{f!r}]", dts, n, device)
+    return n
+
+
+def _check_slabs(slabs, n_particles):
+    """Check a slab table (tuples min_index, max_index, dx, min_weight,
+    cell_offset, particle_offset, n) against a bank of n_particles; returns
+    the number of cells the table spans (max of cell_offset + cells)."""
+    cells = 0
+    for k, s in enumerate(slabs):
+        lo, hi, _dx, _mw, coff, poff, n = s
+        if hi <= lo or coff < 0 or poff < 0 or n < 0 or poff + n > n_particles:
+            raise ValueError(f"slab {k}: {s} does not fit a bank of {n_particles} particles")
+        cells = max(cells, coff + hi - lo)
+    return cells
 
 
 def _bank_struct(bank):
@@ -150,9 +216,16 @@ class Context:
                         collisions=None, options=None, stream=None, allow=()):
         """Device-resident transport (mcgpu_simulate_device) on a dict-of-tensors
         bank; tensors are updated in place.  Returns (stats dict, status)."""
-        b = Bank(_ptr(bank["seed"]), _ptr(bank["x"]), _ptr(bank["mu"]), _ptr(bank["wmc"]),
-                 _ptr(bank["index"]))
-        n = bank["x"].numel()
+        import torch
+        d, m = self.device, max_index - min_index
+        n = check_bank(bank, "bank", device=d)
+        for name, t in (("sigs", sigs), ("absorption_rates", absorption_rates),
+                        ("weights_absorbed", weights_absorbed)):
+            check_tensor(t, name, torch.float64, m, d)
+        check_tensor(result, "result", torch.int8, n, d, optional=True)
+        check_tensor(events, "events", (torch.int64, torch.uint64), n, d, optional=True)
+        check_tensor(collisions, "collisions", (torch.int64, torch.uint64), n, d, optional=True)
+        b = _bank_struct(bank)
         st = Stats()
         opt = options if options is not None else make_options()
         rc = lib().mcgpu_simulate_device(self._h, n, C.byref(b), _ptr(sigs), _ptr(absorption_rates),
@@ -169,8 +242,16 @@ class Context:
                       options=None, stream=None):
         """mcgpu_forward_cells: transport with per-cell (sig_a, sig_i) given
         directly; optionally stores each history's weighted tally."""
-        b = Bank(_ptr(bank["seed"]), _ptr(bank["x"]), _ptr(bank["mu"]), _ptr(bank["wmc"]),
-                 _ptr(bank["index"]))
+        import torch
+        d, m = self.device, max_index - min_index
+        n = check_bank(bank, "bank", device=d)
+        for name, t in (("sig_a", sig_a), ("sig_i", sig_i),
+                        ("weights_absorbed", weights_absorbed)):
+            check_tensor(t, name, torch.float64, m, d)
+        check_tensor(g, "g", torch.float64, m, d, optional=True)
+        check_tensor(history_score, "history_score", torch.float64, n, d, optional=True)
+        check_tensor(result, "result", torch.int8, n, d, optional=True)
+        b = _bank_struct(bank)
         st = Stats()
         opt = options if options is not None else make_options()
         rc = lib().mcgpu_forward_cells(self._h, bank["x"].numel(), C.byref(b), _ptr(sig_a),
@@ -184,8 +265,14 @@ class Context:
                       particle_min_weight, g, grad_sig_a, grad_sig_i, history_score=None,
                       options=None, stream=None):
         """mcgpu_adjoint_cells: adds dT/dsig_a, dT/dsig_i of T = sum g*tally."""
-        b = Bank(_ptr(initial_bank["seed"]), _ptr(initial_bank["x"]), _ptr(initial_bank["mu"]),
-                 _ptr(initial_bank["wmc"]), _ptr(initial_bank["index"]))
+        import torch
+        d, m = self.device, max_index - min_index
+        n = check_bank(initial_bank, "initial_bank", device=d)
+        for name, t in (("sig_a", sig_a), ("sig_i", sig_i), ("g", g),
+                        ("grad_sig_a", grad_sig_a), ("grad_sig_i", grad_sig_i)):
+            check_tensor(t, name, torch.float64, m, d)
+        check_tensor(history_score, "history_score", torch.float64, n, d, optional=True)
+        b = _bank_struct(initial_bank)
         st = Stats()
         opt = options if options is not None else make_options()
         rc = lib().mcgpu_adjoint_cells(self._h, initial_bank["x"].numel(), C.byref(b), _ptr(sig_a),
@@ -199,6 +286,18 @@ class Context:
     def forward_cells_batch(self, slabs, bank, sig_a, sig_i, weights_absorbed, g=None,
                             history_score=None, result=None, options=None, stream=None):
         """mcgpu_forward_cells_batch over slabs (tuples as in Batch)."""
+        import torch
+        d = self.device
+        n = check_bank(bank, "bank", device=d)
+        m = _check_slabs(slabs, n)
+        check_tensor(sig_a, "sig_a", torch.float64, None, d)
+        m = max(m, sig_a.numel())
+        for name, t in (("sig_a", sig_a), ("sig_i", sig_i),
+                        ("weights_absorbed", weights_absorbed)):
+            check_tensor(t, name, torch.float64, m, d)
+        check_tensor(g, "g", torch.float64, m, d, optional=True)
+        check_tensor(history_score, "history_score", torch.float64, n, d, optional=True)
+        check_tensor(result, "result", torch.int8, n, d, optional=True)
         arr = (Slab * len(slabs))(*[Slab(*s) for s in slabs])
         b = _bank_struct(bank)
         st = Stats()
@@ -214,6 +313,16 @@ class Context:
     def adjoint_cells_batch(self, slabs, initial_bank, sig_a, sig_i, g, grad_sig_a, grad_sig_i,
                             history_score=None, options=None, stream=None):
         """mcgpu_adjoint_cells_batch over slabs (tuples as in Batch)."""
+        import torch
+        d = self.device
+        n = check_bank(initial_bank, "initial_bank", device=d)
+        m = _check_slabs(slabs, n)
+        check_tensor(sig_a, "sig_a", torch.float64, None, d)
+        m = max(m, sig_a.numel())
+        for name, t in (("sig_a", sig_a), ("sig_i", sig_i), ("g", g),
+                        ("grad_sig_a", grad_sig_a), ("grad_sig_i", grad_sig_i)):
+            check_tensor(t, name, torch.float64, m, d)
+        check_tensor(history_score, "history_score", torch.float64, n, d, optional=True)
         arr = (Slab * len(slabs))(*[Slab(*s) for s in slabs])
         b = _bank_struct(initial_bank)
         st = Stats()
@@ -268,23 +377,31 @@ class Context:
 
     # ------------------------------------------------------------ helpers
     def rng_real_step(self, seeds, reals, stream=None):
+        import torch
+        check_tensor(seeds, "seeds", (torch.int64, torch.uint64), None, self.device)
+        check_tensor(reals, "reals", torch.float64, seeds.numel(), self.device)
         self.check(lib().mcgpu_rng_real_step(self._h, seeds.numel(), _ptr(seeds), _ptr(reals),
                                              _stream(stream)))
 
     def rng_seed_chain(self, seed, out, stream=None):
+        import torch
+        check_tensor(out, "out", (torch.int64, torch.uint64), None, self.device)
         self.check(lib().mcgpu_rng_seed_chain(self._h, seed, out.numel(), _ptr(out),
                                               _stream(stream)))
 
     def create_particles(self, seed, first, bank, x_ini, wmc, dx, stream=None):
         """mcgpu_create_particles: fill the device bank (dict of tensors, n =
         its length) with source particles first .. first+n-1."""
-        b = Bank(_ptr(bank["seed"]), _ptr(bank["x"]), _ptr(bank["mu"]), _ptr(bank["wmc"]),
-                 _ptr(bank["index"]))
+        check_bank(bank, "bank", device=self.device)
+        b = _bank_struct(bank)
         self.check(lib().mcgpu_create_particles(self._h, seed, first, bank["x"].numel(), x_ini,
                                                 wmc, dx, C.byref(b), _stream(stream)))
 
     def math_eval(self, which, x, out, stream=None):
         w = {"log": 0, "exp": 1}[which]
+        import torch
+        check_tensor(x, "x", torch.float64, None, self.device)
+        check_tensor(out, "out", torch.float64, x.numel(), self.device)
         self.check(lib().mcgpu_math_eval(self._h, w, x.numel(), _ptr(x), _ptr(out),
                                          _stream(stream)))
 
@@ -299,11 +416,20 @@ class Batch:
 
     def __init__(self, ctx, slabs, sigs, absorption_rates, weights, bank, result=None,
                  options=None):
+        import torch
         self.ctx = ctx
+        d = ctx.device
+        n = check_bank(bank, "bank", device=d)
+        m = _check_slabs(slabs, n)
+        check_tensor(weights, "weights", torch.float64, None, d)
+        m = max(m, weights.numel())
+        for name, t in (("sigs", sigs), ("absorption_rates", absorption_rates),
+                        ("weights", weights)):
+            check_tensor(t, name, torch.float64, m, d)
+        check_tensor(result, "result", torch.int8, n, d, optional=True)
         arr = (Slab * len(slabs))(*[Slab(*s) for s in slabs])
         self._keep = (sigs, absorption_rates, weights, bank, result)
-        b = Bank(_ptr(bank["seed"]), _ptr(bank["x"]), _ptr(bank["mu"]), _ptr(bank["wmc"]),
-                 _ptr(bank["index"]))
+        b = _bank_struct(bank)
         self._h = C.c_void_p()
         opt = options if options is not None else make_options()
         # the plan runs on its own stream: finish the torch work that built the inputs

@@ -11,7 +11,7 @@ each history once instead of twice.
 """
 import torch
 
-from . import Context, make_options
+from . import check_bank, check_tensor, make_options
 
 
 class SlabProblem:
@@ -24,6 +24,9 @@ class SlabProblem:
         self.min_index, self.max_index = min_index, max_index
         self.dx, self.min_w = dx, particle_min_weight
         self.options = options if options is not None else make_options()
+        if max_index <= min_index:
+            raise ValueError("max_index must exceed min_index")
+        check_bank(bank, "bank", device=getattr(ctx, "device", None))
 
     @property
     def m(self):
@@ -39,14 +42,22 @@ class SlabTally(torch.autograd.Function):
 
     @staticmethod
     def forward(fctx, sig_a, sig_s, problem, target_hint=None):
-        sig_a = sig_a.detach().contiguous()
-        sig_s = sig_s.detach().contiguous()
+        # The library reads m doubles of each and the backward writes m
+        # doubles of gradient: anything else is rejected before any launch.
+        d = getattr(problem.ctx, "device", None)
+        check_tensor(sig_a, "sig_a", torch.float64, problem.m, d)
+        check_tensor(sig_s, "sig_s", torch.float64, problem.m, d)
+        if target_hint is not None:
+            check_tensor(target_hint, "target_hint", torch.float64, problem.m, d)
+        check_bank(problem.bank, "problem.bank", device=d)
+        sig_a = sig_a.detach()
+        sig_s = sig_s.detach()
         work = {k: v.clone() for k, v in problem.bank.items()}
         tally = torch.zeros(problem.m, dtype=torch.float64, device=sig_a.device)
         score = None
         g = None
         if target_hint is not None:
-            g = target_hint.detach().to(torch.float64).contiguous()
+            g = target_hint.detach()
             score = torch.empty(problem.n, dtype=torch.float64, device=sig_a.device)
         problem.ctx.forward_cells(work, sig_a, sig_s, tally, problem.min_index,
                                   problem.max_index, problem.dx, problem.min_w, g=g,
@@ -62,6 +73,7 @@ class SlabTally(torch.autograd.Function):
         sig_a, sig_s = fctx.saved_tensors
         p = fctx.problem
         g = grad_tally.detach().to(torch.float64).contiguous()
+        check_tensor(g, "grad_output", torch.float64, p.m, getattr(p.ctx, "device", None))
         score = None
         if fctx.hint is not None and torch.equal(g, fctx.hint):
             score = fctx.score
