@@ -152,6 +152,30 @@ int mcgpu_simulate_device(mcgpu_context *ctx, int64_t n, const mcgpu_bank *bank,
                           uint64_t *events, uint64_t *collisions, const mcgpu_options *opt,
                           mcgpu_stats *stats, void *stream);
 
+/*
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code that must not call hipMalloc, such as tight serving loops
+ * or harnesses that forbid allocation).  mcgpu_simulate_device without a
+ * context: the per-call scratch (cell table, exact tally, flags, counters)
+ * lives in a fixed device array of the library, one per GPU, held until the
+ * process ends; the call allocates nothing (no hipMalloc / hipFree, no
+ * events; stats->kernel_ms is 0).  It runs on the current HIP device, on
+ * `stream` (which must belong to that device), with the history kernel only
+ * (options.kernel must be MCGPU_KERNEL_HISTORY), for layers of at most
+ * MCGPU_NOALLOC_MAX_CELLS cells; larger layers, the event kernel or bad
+ * arguments return MCGPU_ERR_ARG.  Results are those of
+ * mcgpu_simulate_device.  All calls on one GPU share the array and each
+ * call clears it first: calls that overlap on one GPU (other threads, or
+ * streams not ordered with each other) race and must not be made.
+ */
+#define MCGPU_NOALLOC_MAX_CELLS 65536
+int mcgpu_simulate_device_noalloc(int64_t n, const mcgpu_bank *bank, const double *sigs,
+                                  const double *absorption_rates, double *weights_absorbed,
+                                  int32_t min_index, int32_t max_index, double dx,
+                                  double particle_min_weight, int8_t *result, uint64_t *events,
+                                  uint64_t *collisions, const mcgpu_options *opt,
+                                  mcgpu_stats *stats, void *stream);
+
 /* ------------------------------------------------------------- batched
  * Many independent slab problems in one persistent launch (a work queue of
  * (slab, particle range) tiles), replayed from a captured hipGraph.  Each

@@ -20,7 +20,7 @@
 /* ---------------------------------------------------------------- helpers */
 
 int mc_fail(mcgpu_context *ctx, int status, const char *what, hipError_t e) {
-  if (ctx) {
+  if (ctx && !ctx->quiet) {
     ctx->err = what ? what : "";
     if (e != hipSuccess) {
       ctx->err += ": ";
@@ -132,11 +132,9 @@ int prepare(mcgpu_context *ctx, int m, const double *d_sigs, const double *d_ar,
   if ((rc = mc_ensure(ctx, &ctx->cells, &ctx->cells_bytes, sizeof(mc_cell) * (size_t)m))) return rc;
   if ((rc = mc_ensure(ctx, &ctx->fx, &ctx->fx_bytes, 16 * (size_t)m))) return rc;
   if ((rc = mc_ensure(ctx, &ctx->flags, &ctx->flags_bytes, 4 * (size_t)m))) return rc;
-  hipLaunchKernelGGL(k_setup, dim3(grid_for(m > 0 ? m : 1, 256)), dim3(256), 0, s, m, d_sigs,
-                     d_ar, (mc_cell *)ctx->cells, (unsigned long long *)ctx->fx,
-                     (uint32_t *)ctx->flags, (mc_counters *)ctx->counters);
-  MC_HIP(ctx, hipGetLastError());
-  return MCGPU_OK;
+  return mc_launch_setup(ctx, m, d_sigs, d_ar, (mc_cell *)ctx->cells,
+                         (unsigned long long *)ctx->fx, (uint32_t *)ctx->flags,
+                         (mc_counters *)ctx->counters, s);
 }
 
 int transport(mcgpu_context *ctx, mc_transport_args &a, const mcgpu_options *opt,
@@ -161,29 +159,13 @@ int transport(mcgpu_context *ctx, mc_transport_args &a, const mcgpu_options *opt
   }
   if (rc) return rc;
   MC_HIP(ctx, hipEventRecord(ctx->ev1, s));
-  const int m = a.L.m;
-  hipLaunchKernelGGL(k_finalize, dim3(grid_for(m, 256)), dim3(256), 0, s, m,
-                     (const unsigned long long *)ctx->fx, (const uint32_t *)ctx->flags, d_weights);
-  MC_HIP(ctx, hipGetLastError());
+  if ((rc = mc_launch_finalize(ctx, a.L.m, a.fx, a.flags, d_weights, s))) return rc;
   MC_HIP(ctx, hipMemcpyAsync(ctx->host_counters, ctx->counters, sizeof(mc_counters),
                              hipMemcpyDeviceToHost, s));
   MC_HIP(ctx, hipStreamSynchronize(s));
-  const mc_counters *hc = (const mc_counters *)ctx->host_counters;
-  if (stats) {
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-    stats->events = hc->events;
-    stats->collisions = hc->collisions;
-    stats->n_left = (int64_t)hc->n_left;
-    stats->n_right = (int64_t)hc->n_right;
-    stats->n_killed = (int64_t)hc->n_killed;
-    stats->kernel_ms = ms;
-  }
-  if (hc->status & 1u)
-    return mc_fail(ctx, MCGPU_ERR_EVENT_CAP, "a history exceeded max_events_per_history");
-  if (hc->status & 2u)
-    return mc_fail(ctx, MCGPU_ERR_RANGE, "a tally term was outside +-2^39");
-  return MCGPU_OK;
+  float ms = 0.f;
+  if (stats) (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+  return mc_status_from_counters(ctx, *(const mc_counters *)ctx->host_counters, ms, stats);
 }
 
 int check_layer(mcgpu_context *ctx, int32_t min_index, int32_t max_index, double dx) {
@@ -193,6 +175,39 @@ int check_layer(mcgpu_context *ctx, int32_t min_index, int32_t max_index, double
 }
 
 } // namespace
+
+int mc_launch_setup(mcgpu_context *ctx, int m, const double *sigs, const double *ar,
+                    mc_cell *cells, unsigned long long *fx, uint32_t *flags, mc_counters *ctr,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(k_setup, dim3(grid_for(m > 0 ? m : 1, 256)), dim3(256), 0, s, m, sigs, ar,
+                     cells, fx, flags, ctr);
+  MC_HIP(ctx, hipGetLastError());
+  return MCGPU_OK;
+}
+
+int mc_launch_finalize(mcgpu_context *ctx, int m, const unsigned long long *fx,
+                       const uint32_t *flags, double *weights, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize, dim3(grid_for(m, 256)), dim3(256), 0, s, m, fx, flags, weights);
+  MC_HIP(ctx, hipGetLastError());
+  return MCGPU_OK;
+}
+
+int mc_status_from_counters(mcgpu_context *ctx, const mc_counters &hc, float kernel_ms,
+                            mcgpu_stats *stats) {
+  if (stats) {
+    stats->events = hc.events;
+    stats->collisions = hc.collisions;
+    stats->n_left = (int64_t)hc.n_left;
+    stats->n_right = (int64_t)hc.n_right;
+    stats->n_killed = (int64_t)hc.n_killed;
+    stats->kernel_ms = kernel_ms;
+  }
+  if (hc.status & 1u)
+    return mc_fail(ctx, MCGPU_ERR_EVENT_CAP, "a history exceeded max_events_per_history");
+  if (hc.status & 2u)
+    return mc_fail(ctx, MCGPU_ERR_RANGE, "a tally term was outside +-2^39");
+  return MCGPU_OK;
+}
 
 int mc_launch_create(mcgpu_context *ctx, uint64_t seed, int64_t first, int64_t n, double x_ini,
                      double wmc, int32_t index, const mcgpu_bank &bank, hipStream_t s) {
@@ -246,6 +261,11 @@ int mcgpu_context_create(int device, mcgpu_context **out) {
   }
   ctx->num_cus = prop.multiProcessorCount;
   ctx->lds_per_block = prop.sharedMemPerBlock;
+  /* library set-up of this device for the no-allocation entry point */
+  if (mc_noalloc_device_setup(device) != MCGPU_OK) {
+    delete ctx;
+    return MCGPU_ERR_HIP;
+  }
   if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
       hipMalloc(&ctx->counters, sizeof(mc_counters)) != hipSuccess ||

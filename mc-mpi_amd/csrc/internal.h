@@ -30,6 +30,7 @@ struct mcgpu_context {
   int device = 0;
   int num_cus = 0;
   size_t lds_per_block = 0;
+  bool quiet = false; /* no error text (a stack context of the no-allocation entry point) */
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::string err;
@@ -143,6 +144,27 @@ static inline int mc_auto_block(K kern, size_t lds, int *per_cu) {
 
 int mc_launch_history(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_options &o,
                       hipStream_t s);
+
+/* The steps of one transport call around the kernel (capi.hip), on buffers
+   the caller provides:
+   mc_launch_setup:    cells[c] = (sigs*ar, sigs*(1-ar)), tally words and
+                       flags of m cells and the counter block zeroed (a
+                       kernel, no memset);
+   mc_launch_finalize: weights[c] += the tally of cell c, rounded once;
+   mc_status_from_counters: stats from the counter block read back (with
+                       the kernel time given) and the call's status. */
+int mc_launch_setup(mcgpu_context *ctx, int m, const double *sigs, const double *ar,
+                    mc_cell *cells, unsigned long long *fx, uint32_t *flags, mc_counters *ctr,
+                    hipStream_t s);
+int mc_launch_finalize(mcgpu_context *ctx, int m, const unsigned long long *fx,
+                       const uint32_t *flags, double *weights, hipStream_t s);
+int mc_status_from_counters(mcgpu_context *ctx, const mc_counters &hc, float kernel_ms,
+                            mcgpu_stats *stats);
+
+/* Device properties and the address of the no-allocation entry point's
+   scratch array on the current device, set up once per device (noalloc.hip;
+   also called by mcgpu_context_create for its device). */
+int mc_noalloc_device_setup(int device);
 int mc_launch_event(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_options &o,
                     hipStream_t s);
 

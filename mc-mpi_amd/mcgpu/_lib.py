@@ -24,6 +24,8 @@ ERR_RANGE = -6
 KERNEL_HISTORY = 0
 KERNEL_EVENT = 1
 
+NOALLOC_MAX_CELLS = 65536  # include/mcgpu.h MCGPU_NOALLOC_MAX_CELLS
+
 # include/mcgpu.h mcgpu_particle (include/types/particle.hpp:7-17, FP64): 40 B
 PARTICLE = np.dtype([("seed", "<u8"), ("x", "<f8"), ("mu", "<f8"), ("wmc", "<f8"),
                      ("index", "<i4"), ("pad_", "<i4")])
@@ -95,6 +97,9 @@ SIGNATURES = {
                              C.POINTER(Options), C.POINTER(Stats)]),
     "mcgpu_simulate_device": (i32, [vp, i64, C.POINTER(Bank), vp, vp, vp, i32, i32, dbl, dbl,
                                     vp, vp, vp, C.POINTER(Options), C.POINTER(Stats), vp]),
+    "mcgpu_simulate_device_noalloc": (i32, [i64, C.POINTER(Bank), vp, vp, vp, i32, i32, dbl,
+                                            dbl, vp, vp, vp, C.POINTER(Options),
+                                            C.POINTER(Stats), vp]),
     "mcgpu_batch_create": (i32, [vp, i32, vp, vp, vp, vp, i64, C.POINTER(Bank), vp,
                                  C.POINTER(Options), C.POINTER(vp)]),
     "mcgpu_batch_run": (i32, [vp, vp, C.POINTER(Stats)]),
