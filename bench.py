@@ -383,13 +383,20 @@ def run_cycle(ctx, args, torch, mcgpu, configs):
                        f"cells, {n} particles from x_ini = 0.70710677, min weight 1e-12, 1e5 "
                        "per cycle, sync; mcgpu_run --ranks R (all ranks on this GPU)",
            "runs": {}}
+    runs = []
     for R in (int(r) for r in args.cycle_ranks.split(",")):
+        runs.append((str(R), R, []))
+        if R == 1:  # the same job with its cycles kept one after the other
+            runs.append(("1_cycle_by_cycle", 1, ["--cycle-by-cycle"]))
+    for key, R, extra in runs:
         with tempfile.TemporaryDirectory() as d:
             cfg = os.path.join(d, "config.yaml")
             with open(cfg, "w") as f:
                 f.write(REFERENCE_JOB_YAML.format(n=n))
-            r = subprocess.run([exe, cfg, "--ranks", str(R)], cwd=d, capture_output=True,
+            r = subprocess.run([exe, cfg, "--ranks", str(R)] + extra, cwd=d, capture_output=True,
                                text=True, timeout=600)
+            with open(os.path.join(d, "out", "weights.csv")) as f:
+                weights_csv = f.read()
         if r.returncode != 0:
             raise RuntimeError(f"mcgpu_run failed: {r.stderr[-500:]}")
         secs = float(r.stdout.split()[0])
@@ -400,13 +407,24 @@ def run_cycle(ctx, args, torch, mcgpu, configs):
         fields = dict(kv.split(" ", 1) for kv in summary[-1].split(", "))
         events = int(fields["events"])
         run = {"events": events, "cycles": int(fields["cycles"]), "wall_s": secs,
-               "kernel_s": float(fields["kernel_ms"]) / 1e3, "events_per_s": events / secs}
+               "kernel_s": float(fields["kernel_ms"]) / 1e3, "events_per_s": events / secs,
+               "weights_csv_sha1": __import__("hashlib").sha1(weights_csv.encode()).hexdigest()}
+        if extra:
+            run["mode"] = "cycle by cycle (--cycle-by-cycle)"
+        elif R == 1:
+            run["mode"] = ("one rank: the cycles' particle sets follow from the counts, so all "
+                           "cycles are transported concurrently, tallies added per cycle in order")
         if n == 100_000_000 and R in REFERENCE_JOB_WALL_S:
             ref = REFERENCE_JOB_WALL_S[R]
             run["reference_published_wall_s"] = ref
             run["speedup_vs_published"] = ref / secs
-        out["runs"][str(R)] = run
-        log(f"cycle, {R} ranks: {run}")
+        out["runs"][key] = run
+        log(f"cycle, {key}: {run}")
+    if "1" in out["runs"] and "1_cycle_by_cycle" in out["runs"]:
+        a, b = out["runs"]["1"], out["runs"]["1_cycle_by_cycle"]
+        out["one_rank_modes_identical"] = (a["weights_csv_sha1"] == b["weights_csv_sha1"] and
+                                           a["events"] == b["events"] and
+                                           a["cycles"] == b["cycles"])
     return out
 
 

@@ -353,12 +353,40 @@ typedef struct mcgpu_cycle_rank {
   int64_t n_remaining;               /* particles left in the bank */
 } mcgpu_cycle_rank;
 
+/* One statistics window of the loop: the Timer::State of the reference's
+   workers (include/timer/timer.hpp:19-29, src/timer.cpp:48-58) and the
+   cycles counted in it (src/worker_sync.cpp:121-133).  All ranks share one
+   launch per cycle, so a window holds for every rank. */
+typedef struct mcgpu_cycle_window {
+  double starttime, endtime; /* wall clock, seconds since the epoch */
+  double time_comp;          /* creation + transport + tally (device time) */
+  double time_send;          /* partition + exchange (device time) */
+  double time_recv;          /* the rest of each cycle: launch overhead, read-back
+                                of the summary, termination test */
+  double time_idle;          /* from the start of the loop to its first cycle */
+  int64_t nb_cycles;
+} mcgpu_cycle_window;
+
+typedef void (*mcgpu_cycle_window_fn)(void *user, const mcgpu_cycle_window *window);
+
 typedef struct mcgpu_cycle_stats {
+  /* out */
   int64_t cycles;
   uint64_t events;
   uint64_t collisions;
   double kernel_ms;    /* the transport launches */
   double wall_ms;      /* the whole loop */
+  /* in: statistics windows, kept as WorkerSync::spin keeps them
+     (src/worker_sync.cpp:25-27,121-133): a window closes after a cycle that
+     ends more than statistics_cycle_time seconds after the window opened,
+     and when the loop ends (the cycle that ends the loop is not counted);
+     on_window (may be NULL) is called with each closed window, in order */
+  double statistics_cycle_time;
+  mcgpu_cycle_window_fn on_window;
+  void *window_user;
+  /* in: 1 = always run the cycles one after the other (see below) */
+  int32_t cycle_by_cycle;
+  int32_t pad_;
 } mcgpu_cycle_stats;
 
 /* Runs cycles until the ranks' nb_disabled add up to nb_particles.
@@ -368,7 +396,19 @@ typedef struct mcgpu_cycle_stats {
    opt: block_size (default 256), particles_per_lane (default 1),
    events_per_refill, grid_blocks and max_events_per_history of the
    transport launch.  Returns MCGPU_ERR_ARG (and leaves the ranks' outputs
-   as of the last cycle) if max_cycles > 0 cycles pass without termination. */
+   as of the last cycle) if max_cycles > 0 cycles pass without termination.
+   stats may be NULL; its statistics_cycle_time / on_window / window_user
+   and cycle_by_cycle fields are read as inputs (zeroed: no windows
+   reported).
+   One rank whose layer has global borders at both ends (and whose bank,
+   source and nb_disabled add up to nb_particles) exchanges nothing, and
+   every particle a cycle transports leaves the bank: which particles each
+   cycle transports then follows from the counts alone.  Unless
+   cycle_by_cycle is set, all its cycles are transported concurrently
+   (batched launches, each cycle with its own exact tally) and the tallies
+   are added to weights_absorbed cycle after cycle, each rounded once:
+   results bit-identical to the cycle-by-cycle loop.  Its statistics
+   windows then cover whole launches (time_send 0). */
 int mcgpu_sync_cycle(mcgpu_context *ctx, int32_t world_size, mcgpu_cycle_rank *ranks,
                      const double *sigs, const double *absorption_rates,
                      double *weights_absorbed, int64_t total_cells,

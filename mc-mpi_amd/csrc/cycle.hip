@@ -304,13 +304,95 @@ inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b
 struct cycle_mem {
   std::vector<void *> dev;
   std::vector<void *> host;
+  std::vector<hipEvent_t> events;
   ~cycle_mem() {
     for (void *p : dev)
       if (p) (void)hipFree(p);
     for (void *p : host)
       if (p) (void)hipHostFree(p);
+    for (hipEvent_t e : events)
+      if (e) (void)hipEventDestroy(e);
   }
 };
+
+/* wall clock in seconds since the epoch (the reference's timer reads
+   MPI_Wtime plus the offset to CLOCK_REALTIME, src/timer.cpp:7-23) */
+double wall_now() {
+  return std::chrono::duration<double>(std::chrono::system_clock::now().time_since_epoch())
+      .count();
+}
+
+/* The statistics windows of WorkerSync::spin (src/worker_sync.cpp:25-27,
+   121-133): each cycle's phase times go to the open window; after a cycle
+   that did not end the loop, the window closes if it is older than
+   statistics_cycle_time, and the next one opens; the last window closes
+   when the loop ends, without counting the cycle that ended it. */
+struct window_log {
+  double period = 0;
+  mcgpu_cycle_window_fn fn = nullptr;
+  void *user = nullptr;
+  mcgpu_cycle_window w{};
+  void open(double t) {
+    w = mcgpu_cycle_window{};
+    w.starttime = t;
+  }
+  void close(double t) {
+    w.endtime = t;
+    if (fn) fn(user, &w);
+  }
+};
+
+/* ---------------------------------------------------------------------
+   One rank: every cycle at once.
+
+   With one rank nothing is exchanged, and when both ends of its layer are
+   global borders every particle a cycle transports leaves the bank for good
+   (an escape through a border or a kill, src/layer.cpp:332-360).  Which
+   particles cycle k transports -- the last min(size, S) of the bank after
+   that cycle's lazy creation (src/layer.cpp:89-121,247-250) -- then follows
+   from the counts alone, never from a transport result.  So the cycles'
+   transports are independent and run concurrently in one batched launch,
+   each cycle a slab of its own with its own exact tally; the tallies are
+   then added to weights_absorbed one cycle after the other, each rounded
+   once (as each Layer::simulate call adds its own).  Histories, counts and
+   weights are those of the cycle-by-cycle loop, bit for bit.
+   The bank's logical order is kept as a stack of segments (parts of the
+   initial bank, chunks of the lazy source); each cycle's particles are laid
+   out contiguously, in bank order, in a work buffer. */
+
+struct one_seg {
+  int kind;          /* 0: initial bank, 1: lazy source */
+  int64_t begin, n;  /* initial-bank index, or position in the source's seed chain */
+};
+
+struct one_cycle {
+  int64_t first, n;              /* the cycle's particles in the work buffer */
+  std::vector<one_seg> pieces;   /* where they come from, in bank order */
+};
+
+/* the cycles' cell tables: the rank's m cells once per cycle */
+__global__ void k_one_cells(int64_t G, int m, const double *sigs, const double *ar,
+                            mc_cell *cells) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < G * m) {
+    const int c = (int)(i % m);
+    cells[i] = mc_make_cell(sigs[c], ar[c]);
+  }
+}
+
+/* weights[c] += the tally of cycle g, rounded once, for g = 0 .. G-1 in order */
+__global__ void k_one_finalize(int m, int64_t G, const unsigned long long *fx,
+                               const uint32_t *flags, double *weights) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= m) return;
+  const int64_t total = G * m;
+  double w = weights[c];
+  for (int64_t g = 0; g < G; ++g) {
+    const int64_t i = g * m + c;
+    w = w + mc_fx_finalize(fx[i], fx[total + i], flags[i]);
+  }
+  weights[c] = w;
+}
 
 int alloc_pool(mcgpu_context *ctx, int64_t slots, pool_k *p, int8_t **result8) {
   MC_HIP(ctx, hipMalloc(&p->seed, 8 * (size_t)slots));
@@ -339,6 +421,267 @@ struct pool_guard {
   ~pool_guard() { free_pool(&p, &result8); }
 };
 
+/* The single-rank path applies when the rank's layer has global borders at
+   both ends and its counts close (bank + source + disabled = nb_particles):
+   the loop then ends exactly when bank and source are empty. */
+bool one_rank_eligible(int R, const mcgpu_cycle_rank *ranks, int64_t nb_particles) {
+  if (R != 1) return false;
+  const mcgpu_cycle_rank &k = ranks[0];
+  return k.left_border && k.right_border &&
+         k.n_particles + k.nb_particles_create + k.nb_disabled == nb_particles;
+}
+
+/* mcgpu_sync_cycle for one eligible rank (see k_one_cells above).  Cycles run
+   in groups bounded by the work buffer (particles) and the per-cycle
+   tallies (cells x cycles); each group is one batched launch. */
+int sync_one_rank(mcgpu_context *ctx, mcgpu_cycle_rank &k, const double *sigs, const double *ar,
+                  double *weights, int64_t S, int64_t max_cycles, const mcgpu_options &o,
+                  window_log &wl, mcgpu_cycle_stats &st, int *status) {
+  hipStream_t s = ctx->stream;
+  const int m = k.max_index - k.min_index;
+  const int block = o.block_size > 0 ? o.block_size : 256;
+  const int ppl = o.particles_per_lane > 0 ? o.particles_per_lane : 16;
+  const int64_t T = (int64_t)block * ppl;
+  const uint64_t max_events = o.max_events_per_history ? o.max_events_per_history : (1ULL << 27);
+  mc_batched_shape sh;
+  int rc = mc_batched_plan_shape(ctx, m, block, o.grid_blocks, o.events_per_refill, &sh);
+  if (rc) return rc;
+
+  const int64_t n0 = k.n_particles;
+  const int64_t all = n0 + k.nb_particles_create;
+  /* work buffer and per-cycle tallies of one group */
+  const int64_t pcap = std::max(S, std::min<int64_t>(all, (int64_t)1 << 26));
+  const int64_t gcap = std::max<int64_t>(1, ((int64_t)1 << 23) / m);
+  const int64_t max_tiles = pcap / T + gcap + 1;
+
+  cycle_mem mem;
+  auto dalloc = [&](void **p, size_t bytes) -> int {
+    MC_HIP(ctx, hipMalloc(p, bytes));
+    mem.dev.push_back(*p);
+    return MCGPU_OK;
+  };
+  double *d_in, *d_w;
+  mc_cell *d_cells;
+  unsigned long long *d_fx;
+  uint32_t *d_flags;
+  mc_counters *d_ctr;
+  mc_slab_k *d_slabs;
+  mc_tile_k *d_tiles;
+  void *h_ctr;
+  if ((rc = dalloc((void **)&d_in, 16 * (size_t)m)) || (rc = dalloc((void **)&d_w, 8 * (size_t)m)) ||
+      (rc = dalloc((void **)&d_cells, sizeof(mc_cell) * (size_t)(gcap * m))) ||
+      (rc = dalloc((void **)&d_fx, 16 * (size_t)(gcap * m))) ||
+      (rc = dalloc((void **)&d_flags, 4 * (size_t)(gcap * m))) ||
+      (rc = dalloc((void **)&d_ctr, sizeof(mc_counters))) ||
+      (rc = dalloc((void **)&d_slabs, sizeof(mc_slab_k) * (size_t)gcap)) ||
+      (rc = dalloc((void **)&d_tiles, sizeof(mc_tile_k) * (size_t)max_tiles)))
+    return rc;
+  MC_HIP(ctx, hipHostMalloc(&h_ctr, sizeof(mc_counters), hipHostMallocDefault));
+  mem.host.push_back(h_ctr);
+  hipEvent_t ev_start, ev_comp;
+  MC_HIP(ctx, hipEventCreate(&ev_start));
+  mem.events.push_back(ev_start);
+  MC_HIP(ctx, hipEventCreate(&ev_comp));
+  mem.events.push_back(ev_comp);
+
+  MC_HIP(ctx, hipMemcpyAsync(d_in, sigs + k.cell_offset, 8 * (size_t)m, hipMemcpyHostToDevice, s));
+  MC_HIP(ctx, hipMemcpyAsync(d_in + m, ar + k.cell_offset, 8 * (size_t)m, hipMemcpyHostToDevice,
+                             s));
+  MC_HIP(ctx, hipMemcpyAsync(d_w, weights + k.cell_offset, 8 * (size_t)m, hipMemcpyHostToDevice,
+                             s));
+  hipLaunchKernelGGL(k_one_cells, dim3(blocks_for(gcap * m, 256)), dim3(256), 0, s, gcap, m, d_in,
+                     d_in + m, d_cells);
+  MC_HIP(ctx, hipGetLastError());
+
+  /* the initial bank, host AoS -> device SoA */
+  pool_guard init, work;
+  double wmax = k.nb_particles_create > 0 ? std::fabs(k.wmc) : 0.0;
+  if (n0 > 0) {
+    if ((rc = alloc_pool(ctx, n0, &init.p, nullptr))) return rc;
+    std::vector<uint64_t> hs(n0);
+    std::vector<double> hx(n0), hm(n0), hw(n0);
+    std::vector<int32_t> hi(n0);
+    for (int64_t i = 0; i < n0; ++i) {
+      const mcgpu_particle &q = k.particles[i];
+      hs[i] = q.seed;
+      hx[i] = q.x;
+      hm[i] = q.mu;
+      hw[i] = q.wmc;
+      hi[i] = q.index;
+      wmax = std::max(wmax, std::fabs(q.wmc));
+    }
+    MC_HIP(ctx, hipMemcpyAsync(init.p.seed, hs.data(), 8 * n0, hipMemcpyHostToDevice, s));
+    MC_HIP(ctx, hipMemcpyAsync(init.p.x, hx.data(), 8 * n0, hipMemcpyHostToDevice, s));
+    MC_HIP(ctx, hipMemcpyAsync(init.p.mu, hm.data(), 8 * n0, hipMemcpyHostToDevice, s));
+    MC_HIP(ctx, hipMemcpyAsync(init.p.wmc, hw.data(), 8 * n0, hipMemcpyHostToDevice, s));
+    MC_HIP(ctx, hipMemcpyAsync(init.p.index, hi.data(), 4 * n0, hipMemcpyHostToDevice, s));
+    MC_HIP(ctx, hipStreamSynchronize(s)); /* the host vectors go out of scope */
+  }
+  if ((rc = alloc_pool(ctx, pcap, &work.p, nullptr))) return rc;
+  const int32_t src_index = k.nb_particles_create > 0 ? (int32_t)(k.x_ini / k.dx) : 0;
+
+  wl.open(wall_now()); /* the loop starts (the reference's spin, src/worker_sync.cpp:26) */
+  /* the bank: segments in bank order (bottom first) */
+  std::vector<one_seg> stack;
+  if (n0 > 0) stack.push_back({0, 0, n0});
+  int64_t size = n0, create = k.nb_particles_create, chain = 0;
+  int64_t win_cycles = 0;
+  bool first_group = true;
+  std::vector<mc_slab_k> h_slabs;
+  std::vector<mc_tile_k> h_tiles;
+  while (size > 0 || create > 0) {
+    if (max_cycles > 0 && st.cycles >= max_cycles) {
+      *status = mc_fail(ctx, MCGPU_ERR_ARG, "sync_cycle: no termination within max_cycles");
+      break;
+    }
+    const double t_group = wall_now();
+    if (first_group) {
+      wl.w.time_idle += t_group - wl.w.starttime;
+      first_group = false;
+    }
+    /* ---- the group's cycles: creation and the last n of the bank each */
+    std::vector<one_cycle> cyc;
+    int64_t used = 0;
+    while ((size > 0 || create > 0) && (int64_t)cyc.size() < gcap && used + S <= pcap &&
+           (max_cycles <= 0 || st.cycles + (int64_t)cyc.size() < max_cycles)) {
+      if (size < S && create > 0) { /* src/layer.cpp:89-121,247-248 */
+        const int64_t n_new = std::min(create, std::max(MAX_PARTICLES_VECTOR - size, S));
+        stack.push_back({1, chain, n_new});
+        chain += n_new;
+        create -= n_new;
+        size += n_new;
+      }
+      one_cycle c;
+      c.first = used;
+      c.n = std::min(size, S);
+      int64_t need = c.n;
+      while (need > 0) {
+        one_seg &t = stack.back();
+        const int64_t take = std::min(need, t.n);
+        c.pieces.push_back({t.kind, t.begin + t.n - take, take});
+        t.n -= take;
+        need -= take;
+        if (t.n == 0) stack.pop_back();
+      }
+      std::reverse(c.pieces.begin(), c.pieces.end());
+      size -= c.n;
+      used += c.n;
+      cyc.push_back(std::move(c));
+    }
+    const int64_t G = (int64_t)cyc.size();
+    if (G == 0) break;
+    /* ---- lay out the particles, tables, one launch, weights in cycle order */
+    MC_HIP(ctx, hipEventRecord(ev_start, s));
+    h_slabs.assign(G, mc_slab_k{});
+    h_tiles.clear();
+    for (int64_t g = 0; g < G; ++g) {
+      int64_t pos = cyc[g].first;
+      for (const one_seg &q : cyc[g].pieces) {
+        if (q.kind == 0) {
+          MC_HIP(ctx, hipMemcpyAsync(work.p.seed + pos, init.p.seed + q.begin, 8 * q.n,
+                                     hipMemcpyDeviceToDevice, s));
+          MC_HIP(ctx, hipMemcpyAsync(work.p.x + pos, init.p.x + q.begin, 8 * q.n,
+                                     hipMemcpyDeviceToDevice, s));
+          MC_HIP(ctx, hipMemcpyAsync(work.p.mu + pos, init.p.mu + q.begin, 8 * q.n,
+                                     hipMemcpyDeviceToDevice, s));
+          MC_HIP(ctx, hipMemcpyAsync(work.p.wmc + pos, init.p.wmc + q.begin, 8 * q.n,
+                                     hipMemcpyDeviceToDevice, s));
+          MC_HIP(ctx, hipMemcpyAsync(work.p.index + pos, init.p.index + q.begin, 4 * q.n,
+                                     hipMemcpyDeviceToDevice, s));
+        } else {
+          const mcgpu_bank b{work.p.seed + pos, work.p.x + pos, work.p.mu + pos, work.p.wmc + pos,
+                             work.p.index + pos};
+          if ((rc = mc_launch_create(ctx, k.seed, q.begin, q.n, k.x_ini, k.wmc, src_index, b, s)))
+            return rc;
+        }
+        pos += q.n;
+      }
+      mc_slab_k &sl = h_slabs[g];
+      sl.index_start = k.min_index;
+      sl.m = m;
+      sl.dx = k.dx;
+      sl.min_weight = k.particle_min_weight;
+      sl.cell_off = g * m;
+      const int64_t end = cyc[g].first + cyc[g].n;
+      for (int64_t p0 = cyc[g].first; p0 < end; p0 += T) {
+        mc_tile_k t;
+        t.slab = (int32_t)g;
+        t.pad_ = 0;
+        t.begin = p0;
+        t.end = std::min(end, p0 + T);
+        h_tiles.push_back(t);
+      }
+    }
+    MC_HIP(ctx, hipMemcpyAsync(d_slabs, h_slabs.data(), sizeof(mc_slab_k) * G,
+                               hipMemcpyHostToDevice, s));
+    MC_HIP(ctx, hipMemcpyAsync(d_tiles, h_tiles.data(), sizeof(mc_tile_k) * h_tiles.size(),
+                               hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_cyc_clear, dim3(blocks_for(G * m, 256)), dim3(256), 0, s, G * m, d_fx,
+                       d_flags, d_ctr);
+    MC_HIP(ctx, hipGetLastError());
+    const mcgpu_bank bank{work.p.seed, work.p.x, work.p.mu, work.p.wmc, work.p.index};
+    MC_HIP(ctx, hipEventRecord(ctx->ev0, s));
+    if ((rc = mc_launch_batched(ctx, sh, d_slabs, d_tiles, (int32_t)h_tiles.size(), bank, d_cells,
+                                d_fx, G * m, d_flags, nullptr, d_ctr, max_events,
+                                wmax < 0x1p-24, s)))
+      return rc;
+    MC_HIP(ctx, hipEventRecord(ctx->ev1, s));
+    hipLaunchKernelGGL(k_one_finalize, dim3(blocks_for(m, 256)), dim3(256), 0, s, m, G,
+                       (const unsigned long long *)d_fx, (const uint32_t *)d_flags, d_w);
+    MC_HIP(ctx, hipGetLastError());
+    MC_HIP(ctx, hipEventRecord(ev_comp, s));
+    MC_HIP(ctx, hipMemcpyAsync(h_ctr, d_ctr, sizeof(mc_counters), hipMemcpyDeviceToHost, s));
+    MC_HIP(ctx, hipStreamSynchronize(s)); /* also: h_slabs / h_tiles reusable */
+    const mc_counters *hc = reinterpret_cast<const mc_counters *>(h_ctr);
+    float ms = 0.f, comp_ms = 0.f;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    (void)hipEventElapsedTime(&comp_ms, ev_start, ev_comp);
+    st.kernel_ms += ms;
+    st.events += hc->events;
+    st.collisions += hc->collisions;
+    st.cycles += G;
+    k.n_left += (int64_t)hc->n_left;
+    k.n_right += (int64_t)hc->n_right;
+    k.n_killed += (int64_t)hc->n_killed;
+    k.nb_disabled += used; /* every transported particle leaves through a border or is killed */
+    {
+      const double t_end = wall_now();
+      wl.w.time_comp += comp_ms * 1e-3;
+      wl.w.time_recv += std::max(0.0, (t_end - t_group) - comp_ms * 1e-3);
+    }
+    if (hc->status & 1u) {
+      *status = mc_fail(ctx, MCGPU_ERR_EVENT_CAP, "a history exceeded max_events_per_history");
+      break;
+    }
+    if (hc->status & 2u) {
+      *status = mc_fail(ctx, MCGPU_ERR_RANGE, "a tally term was outside +-2^39");
+      break;
+    }
+    /* statistics windows: the group's cycles as a unit; the cycle that ends
+       the loop is not counted */
+    const bool ended = size == 0 && create == 0;
+    win_cycles += ended ? G - 1 : G;
+    if (!ended) {
+      const double t = wall_now();
+      if (t > wl.w.starttime + wl.period) {
+        wl.w.nb_cycles = win_cycles;
+        wl.close(t);
+        wl.open(t);
+        win_cycles = 0;
+      }
+    }
+  }
+  wl.w.nb_cycles = win_cycles;
+  wl.close(wall_now());
+  MC_HIP(ctx, hipMemcpyAsync(weights + k.cell_offset, d_w, 8 * (size_t)m, hipMemcpyDeviceToHost,
+                             s));
+  MC_HIP(ctx, hipStreamSynchronize(s));
+  k.nb_particles_create = create;
+  k.seed = mc_seed_chain_at(k.seed, (uint64_t)chain);
+  k.n_remaining = size;
+  return MCGPU_OK;
+}
+
 } // namespace
 
 extern "C" int mcgpu_sync_cycle(mcgpu_context *ctx, int32_t world_size, mcgpu_cycle_rank *ranks,
@@ -350,6 +693,14 @@ extern "C" int mcgpu_sync_cycle(mcgpu_context *ctx, int32_t world_size, mcgpu_cy
   if (!ctx) return MCGPU_ERR_ARG;
   ctx->err.clear();
   const auto t_start = std::chrono::steady_clock::now();
+  window_log wl;
+  int32_t cycle_by_cycle = 0;
+  if (stats) {
+    wl.period = stats->statistics_cycle_time;
+    wl.fn = stats->on_window;
+    wl.user = stats->window_user;
+    cycle_by_cycle = stats->cycle_by_cycle;
+  }
   const int R = world_size;
   const int64_t S = nb_particles_per_cycle;
   if (R < 1 || !ranks || !sigs || !absorption_rates || !weights_absorbed || total_cells <= 0 ||
@@ -380,6 +731,27 @@ extern "C" int mcgpu_sync_cycle(mcgpu_context *ctx, int32_t world_size, mcgpu_cy
   const int ppl = o.particles_per_lane > 0 ? o.particles_per_lane : 1;
   const int64_t T = (int64_t)block * ppl; /* particles per transport tile */
   const uint64_t max_events = o.max_events_per_history ? o.max_events_per_history : (1ULL << 27);
+  if (one_rank_eligible(R, ranks, nb_particles) && !cycle_by_cycle) {
+    mcgpu_cycle_rank &k = ranks[0];
+    k.n_left = k.n_right = k.n_killed = 0;
+    k.n_remaining = k.n_particles;
+    mcgpu_cycle_stats st{};
+    int status = MCGPU_OK;
+    int rc = sync_one_rank(ctx, k, sigs, absorption_rates, weights_absorbed, S, max_cycles, o, wl,
+                           st, &status);
+    if (rc) return rc;
+    st.wall_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
+            .count();
+    if (stats) {
+      st.statistics_cycle_time = wl.period;
+      st.on_window = wl.fn;
+      st.window_user = wl.user;
+      st.cycle_by_cycle = cycle_by_cycle;
+      *stats = st;
+    }
+    return status;
+  }
   mc_batched_shape sh;
   int rc = mc_batched_plan_shape(ctx, max_m, block, o.grid_blocks, o.events_per_refill, &sh);
   if (rc) return rc;
@@ -548,14 +920,32 @@ extern "C" int mcgpu_sync_cycle(mcgpu_context *ctx, int32_t world_size, mcgpu_cy
     }
   }
 
+  /* cycle phase boundaries: start, end of the tally (computation), end of
+     the exchange (send) */
+  hipEvent_t ev_start, ev_comp, ev_send;
+  MC_HIP(ctx, hipEventCreate(&ev_start));
+  mem.events.push_back(ev_start);
+  MC_HIP(ctx, hipEventCreate(&ev_comp));
+  mem.events.push_back(ev_comp);
+  MC_HIP(ctx, hipEventCreate(&ev_send));
+  mem.events.push_back(ev_send);
+
   int64_t disabled_total = 0;
   for (int r = 0; r < R; ++r) disabled_total += ranks[r].nb_disabled;
   mcgpu_cycle_stats st{};
   int status = MCGPU_OK;
+  wl.open(wall_now());
+  bool first_cycle = true;
+  int64_t win_cycles = 0;
   while (disabled_total != nb_particles) {
     if (max_cycles > 0 && st.cycles >= max_cycles) {
       status = mc_fail(ctx, MCGPU_ERR_ARG, "sync_cycle: no termination within max_cycles");
       break;
+    }
+    const double t_cycle = wall_now();
+    if (first_cycle) {
+      wl.w.time_idle += t_cycle - wl.w.starttime;
+      first_cycle = false;
     }
     /* room for this cycle */
     bool grow = false;
@@ -566,6 +956,8 @@ extern "C" int mcgpu_sync_cycle(mcgpu_context *ctx, int32_t world_size, mcgpu_cy
         grow = true;
       }
     if (grow && (rc = build_pool(want))) return rc;
+    const double t_grown = wall_now();
+    MC_HIP(ctx, hipEventRecord(ev_start, s));
 
     /* 1. lazy creation (src/layer.cpp:247-248, :89-121) */
     int64_t n_tiles = 0, n_chunks = 0;
@@ -628,6 +1020,7 @@ extern "C" int mcgpu_sync_cycle(mcgpu_context *ctx, int32_t world_size, mcgpu_cy
     hipLaunchKernelGGL(k_cyc_finalize, dim3(blocks_for(tc, 256)), dim3(256), 0, s, tc,
                        (const unsigned long long *)d_fx, (const uint32_t *)d_flags, d_w);
     MC_HIP(ctx, hipGetLastError());
+    MC_HIP(ctx, hipEventRecord(ev_comp, s));
     if (n_chunks > 0) {
       hipLaunchKernelGGL(k_cyc_count, dim3((unsigned)n_chunks), dim3(CHUNK), 0, s,
                          (const chunk_k *)d_chunks, (const int8_t *)pool.result8, d_cnt);
@@ -648,12 +1041,23 @@ extern "C" int mcgpu_sync_cycle(mcgpu_context *ctx, int32_t world_size, mcgpu_cy
     hipLaunchKernelGGL(k_cyc_exchange, dim3(blocks_for(2 * S, 256), R), dim3(256), 0, s,
                        (const rank_in *)d_rin, (const rank_sum *)d_sum, pool.p, esc.p);
     MC_HIP(ctx, hipGetLastError());
+    MC_HIP(ctx, hipEventRecord(ev_send, s));
     MC_HIP(ctx, hipMemcpyAsync(h_sum, d_sum, sizeof(rank_sum) * R, hipMemcpyDeviceToHost, s));
     MC_HIP(ctx, hipMemcpyAsync(h_ctr, d_ctr, sizeof(mc_counters), hipMemcpyDeviceToHost, s));
     MC_HIP(ctx, hipStreamSynchronize(s));
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
     st.kernel_ms += ms;
+    {
+      float comp_ms = 0.f, send_ms = 0.f;
+      (void)hipEventElapsedTime(&comp_ms, ev_start, ev_comp);
+      (void)hipEventElapsedTime(&send_ms, ev_comp, ev_send);
+      const double comp = (t_grown - t_cycle) + comp_ms * 1e-3, send = send_ms * 1e-3;
+      const double t_end = wall_now();
+      wl.w.time_comp += comp;
+      wl.w.time_send += send;
+      wl.w.time_recv += std::max(0.0, (t_end - t_cycle) - comp - send);
+    }
     st.events += h_ctr->events;
     st.collisions += h_ctr->collisions;
     ++st.cycles;
@@ -675,7 +1079,18 @@ extern "C" int mcgpu_sync_cycle(mcgpu_context *ctx, int32_t world_size, mcgpu_cy
       status = mc_fail(ctx, MCGPU_ERR_RANGE, "a tally term was outside +-2^39");
       break;
     }
+    if (disabled_total == nb_particles) break; /* the ending cycle is not counted */
+    const double t = wall_now();
+    if (t > wl.w.starttime + wl.period) {
+      wl.w.nb_cycles = win_cycles;
+      wl.close(t);
+      wl.open(t);
+      win_cycles = 0;
+    }
+    ++win_cycles;
   }
+  wl.w.nb_cycles = win_cycles;
+  wl.close(wall_now());
 
   MC_HIP(ctx, hipMemcpyAsync(weights_absorbed, d_w, 8 * (size_t)tc, hipMemcpyDeviceToHost, s));
   MC_HIP(ctx, hipStreamSynchronize(s));
@@ -686,6 +1101,12 @@ extern "C" int mcgpu_sync_cycle(mcgpu_context *ctx, int32_t world_size, mcgpu_cy
   }
   st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
                    .count();
-  if (stats) *stats = st;
+  if (stats) {
+    st.statistics_cycle_time = wl.period;
+    st.on_window = wl.fn;
+    st.window_user = wl.user;
+    st.cycle_by_cycle = cycle_by_cycle;
+    *stats = st;
+  }
   return status;
 }

@@ -19,6 +19,27 @@ double now() {
       .count();
 }
 
+/* wall clock, seconds since the epoch (the reference's timer: MPI_Wtime
+   plus its offset to CLOCK_REALTIME, src/timer.cpp:7-23) */
+double wall_now() {
+  return std::chrono::duration<double>(std::chrono::system_clock::now().time_since_epoch())
+      .count();
+}
+
+/* mcgpu_cycle_stats::on_window: every rank gets the window */
+void on_window(void *user, const mcgpu_cycle_window *w) {
+  auto *all = static_cast<std::vector<std::vector<TimerWindow>> *>(user);
+  TimerWindow t;
+  t.starttime = w->starttime;
+  t.endtime = w->endtime;
+  t.time_comp = w->time_comp;
+  t.time_send = w->time_send;
+  t.time_recv = w->time_recv;
+  t.time_idle = w->time_idle;
+  t.nb_cycles = w->nb_cycles;
+  for (auto &v : *all) v.push_back(t);
+}
+
 /* One MPI_Sendrecv pair of src/worker_sync.cpp:55-109 between ranks r and
    r+1: r's right escapes are appended to r+1's particles and r+1's left
    escapes to r's, each in the sender's order; the sent buffers are cleared. */
@@ -41,9 +62,17 @@ SyncCycle::SyncCycle(const MCMPIOptions &opt) : options(opt) {
                                       opt.nb_cells, (int)opt.nb_particles,
                                       opt.particle_min_weight));
   times.assign(opt.world_size, RankTimes{});
+  windows.assign(opt.world_size, {});
+}
+
+void SyncCycle::check_usable() const {
+  if (!usable_)
+    throw Error(MCGPU_ERR_ARG, "SyncCycle: a device-resident spin() failed; the layers' banks "
+                               "were not brought back, so this run's state is undefined");
 }
 
 long long SyncCycle::spin(long long max_cycles) {
+  check_usable();
   if (device_resident && options.nb_particles_per_cycle <= MAX_PARTICLES_VECTOR)
     return spin_device(max_cycles);
   return spin_host(max_cycles);
@@ -81,6 +110,10 @@ long long SyncCycle::spin_device(long long max_cycles) {
     k.nb_disabled = L.nb_disabled;
   }
   mcgpu_cycle_stats st{};
+  st.statistics_cycle_time = options.statistics_cycle_time;
+  st.on_window = on_window;
+  st.window_user = &windows;
+  st.cycle_by_cycle = cycle_by_cycle ? 1 : 0;
   mcgpu_context *ctx = default_context();
   const int rc = mcgpu_sync_cycle(ctx, R, rk.data(), sigs.data(), ar.data(), w.data(), total,
                                   options.nb_particles_per_cycle, options.nb_particles, max_cycles,
@@ -102,40 +135,79 @@ long long SyncCycle::spin_device(long long max_cycles) {
   events += st.events;
   nb_cycles += st.cycles;
   elapsed = st.wall_ms * 1e-3;
-  if (rc != MCGPU_OK)
+  if (rc != MCGPU_OK) {
+    usable_ = false;
     throw Error(rc, std::string("mcgpu_sync_cycle: ") + mcgpu_status_string(rc) + ": " +
                         mcgpu_context_error(ctx));
+  }
   return st.cycles;
 }
 
 long long SyncCycle::spin_host(long long max_cycles) {
   const int R = options.world_size;
   const double t_start = now();
+  /* statistics windows (src/worker_sync.cpp:25-27,121-133), per rank */
+  std::vector<TimerWindow> win(R);
+  for (TimerWindow &w : win) w.starttime = wall_now();
+  long long win_cycles = 0;
+  bool first_cycle = true;
+  auto close_windows = [&](double t) {
+    for (int r = 0; r < R; ++r) {
+      win[r].endtime = t;
+      win[r].nb_cycles = win_cycles;
+      windows[r].push_back(win[r]);
+      win[r] = TimerWindow{};
+      win[r].starttime = t;
+    }
+    win_cycles = 0;
+  };
   long long cycles = 0;
-  while (true) {
-    if (max_cycles > 0 && cycles >= max_cycles)
-      throw Error(MCGPU_ERR_ARG, "SyncCycle::spin: no termination within max_cycles");
-    for (int r = 0; r < R; ++r) {
-      const double t0 = now();
-      const unsigned long long before = layers[r].total_stats.events;
-      layers[r].simulate(options.nb_particles_per_cycle, options.nthread);
-      events += layers[r].total_stats.events - before;
-      times[r].comp += now() - t0;
+  try {
+    while (true) {
+      if (max_cycles > 0 && cycles >= max_cycles)
+        throw Error(MCGPU_ERR_ARG, "SyncCycle::spin: no termination within max_cycles");
+      const double tw0 = wall_now();
+      if (first_cycle) {
+        for (TimerWindow &w : win) w.time_idle += tw0 - w.starttime;
+        first_cycle = false;
+      }
+      std::vector<double> comp(R);
+      for (int r = 0; r < R; ++r) {
+        const double t0 = now();
+        const unsigned long long before = layers[r].total_stats.events;
+        layers[r].simulate(options.nb_particles_per_cycle, options.nthread);
+        events += layers[r].total_stats.events - before;
+        comp[r] = now() - t0;
+        times[r].comp += comp[r];
+      }
+      const double t1 = now();
+      for (int r = 1; r + 1 < R; r += 2) sync_pair(layers[r], layers[r + 1]);
+      for (int r = 0; r + 1 < R; r += 2) sync_pair(layers[r], layers[r + 1]);
+      const double t2 = now();
+      ++cycles;
+      long long disabled = 0;
+      for (int r = 0; r < R; ++r) disabled += layers[r].nb_disabled;
+      const double t3 = now();
+      const double tw1 = wall_now();
+      for (int r = 0; r < R; ++r) {
+        times[r].send += t2 - t1;
+        times[r].recv += t3 - t2;
+        win[r].time_comp += comp[r];
+        win[r].time_recv += t3 - t2;
+        win[r].time_send += std::max(0.0, (tw1 - tw0) - comp[r] - (t3 - t2));
+      }
+      if (disabled == options.nb_particles) break; /* the ending cycle is not counted */
+      const double t = wall_now();
+      if (t > win[0].starttime + options.statistics_cycle_time) close_windows(t);
+      ++win_cycles;
     }
-    const double t1 = now();
-    for (int r = 1; r + 1 < R; r += 2) sync_pair(layers[r], layers[r + 1]);
-    for (int r = 0; r + 1 < R; r += 2) sync_pair(layers[r], layers[r + 1]);
-    const double t2 = now();
-    ++cycles;
-    long long disabled = 0;
-    for (int r = 0; r < R; ++r) disabled += layers[r].nb_disabled;
-    const double t3 = now();
-    for (int r = 0; r < R; ++r) {
-      times[r].send += t2 - t1;
-      times[r].recv += t3 - t2;
-    }
-    if (disabled == options.nb_particles) break;
+  } catch (...) {
+    close_windows(wall_now());
+    nb_cycles += cycles;
+    elapsed = now() - t_start;
+    throw;
   }
+  close_windows(wall_now());
   nb_cycles += cycles;
   elapsed = now() - t_start;
   return cycles;
@@ -152,28 +224,14 @@ std::vector<real_t> SyncCycle::gather_weights_absorbed(std::vector<int> *displs)
 }
 
 void SyncCycle::dump(const std::string &folder, const std::string &wa_path) const {
+  check_usable();
   mkdir(folder.c_str(), 0777);
   const std::string cfg = folder + "/config.yaml";
   FILE *f = fopen(cfg.c_str(), "w");
   if (!f) throw Error(MCGPU_ERR_ARG, "cannot write " + cfg);
   char host[1000] = "";
   gethostname(host, sizeof host);
-  fprintf(f, "# %s\n", "Read from config");
-  fprintf(f, "%s: %d\n", "nb_cells", options.nb_cells);
-  fprintf(f, "%s: %.18e\n", "x_min", options.x_min);
-  fprintf(f, "%s: %.18e\n", "x_max", options.x_max);
-  fprintf(f, "%s: %.18e\n", "x_ini", options.x_ini);
-  fprintf(f, "%s: %.18e\n", "particle_min_weight", options.particle_min_weight);
-  fprintf(f, "%s: %d\n", "nb_particles", (int)options.nb_particles);
-  fprintf(f, "%s: %d\n", "buffer_size", 0);
-  fprintf(f, "%s: %.18e\n", "cycle_time", options.cycle_time);
-  fprintf(f, "%s: %d\n", "nb_particles_per_cycle", options.nb_particles_per_cycle);
-  fprintf(f, "%s: %d\n", "nthread", options.nthread);
-  fprintf(f, "%s: %.18e\n", "statistics_cycle_time", options.statistics_cycle_time);
-  fprintf(f, "\n");
-  fprintf(f, "# %s\n", "Other values");
-  fprintf(f, "%s: %d\n", "world_size", options.world_size);
-  fprintf(f, "%s: %s\n", "hostname", host);
+  write_config_yaml<real_t>(f, options, host);
   fclose(f);
 
   std::vector<int> displs;
@@ -181,30 +239,21 @@ void SyncCycle::dump(const std::string &folder, const std::string &wa_path) cons
   const std::string wpath = folder + "/weights.csv";
   f = fopen(wpath.c_str(), "w");
   if (!f) throw Error(MCGPU_ERR_ARG, "cannot write " + wpath);
-  fprintf(f, "proc, x, weight\n");
-  const real_t dx = layers[0].dx;
-  int proc = 0;
-  for (int i = 0; i < (int)w.size(); ++i) {
-    if (proc < options.world_size - 1 && displs[proc + 1] == i) proc++;
-    fprintf(f, "%d, %.18e, %.18e\n", proc, dx * (i + 0.5), w[i] / dx);
-  }
+  write_weights_csv<real_t>(f, w.data(), (int)w.size(), displs.data(), options.world_size,
+                            layers[0].dx);
   fclose(f);
 
   f = fopen(wa_path.c_str(), "w");
   if (!f) throw Error(MCGPU_ERR_ARG, "cannot write " + wa_path);
   const Layer &L0 = layers[0];
-  for (int i = 0; i < (int)L0.weights_absorbed.size(); ++i)
-    fprintf(f, "%.4e %.3e\n", L0.x_min + (i * L0.dx) + 0.5 * L0.dx, L0.weights_absorbed[i] / L0.dx);
+  write_wa<real_t>(f, L0.weights_absorbed.data(), (int)L0.weights_absorbed.size(), L0.x_min,
+                   L0.dx);
   fclose(f);
 
   const std::string spath = folder + "/stats.csv";
   f = fopen(spath.c_str(), "w");
   if (!f) throw Error(MCGPU_ERR_ARG, "cannot write " + spath);
-  fprintf(f, "rank, %s, %s, %s, %s, %s, %s, nb_cycles, \n", "starttime", "endtime", "time_comp",
-          "time_send", "time_recv", "time_idle");
-  for (int r = 0; r < options.world_size; ++r)
-    fprintf(f, "%d, %.18e, %.18e, %.18e, %.18e, %.18e, %.18e, %lld, \n", r, 0.0, elapsed,
-            times[r].comp, times[r].send, times[r].recv, 0.0, nb_cycles);
+  write_stats_csv(f, windows);
   fclose(f);
 }
 

@@ -16,8 +16,11 @@
  * dump() writes the reference's output files (src/worker.cpp:37-61):
  * folder/config.yaml (src/worker.cpp:218-241), folder/weights.csv
  * (src/worker.cpp:287-313), WA.out of rank 0's layer (src/layer.cpp:363-380)
- * and folder/stats.csv (src/worker.cpp:63-181; one row per rank covering the
- * whole run, with time_idle 0).
+ * and folder/stats.csv (src/worker.cpp:63-181): one row per rank and
+ * statistics window, the windows kept as src/worker_sync.cpp:25-27,121-133
+ * keeps them (a window closes after a cycle ending more than
+ * statistics_cycle_time seconds after it opened, and at the end, the ending
+ * cycle not counted).
  */
 #ifndef MCGPU_CYCLE_HPP
 #define MCGPU_CYCLE_HPP
@@ -25,6 +28,7 @@
 #include <string>
 #include <vector>
 
+#include "formats.hpp"
 #include "layer.hpp"
 #include "yaml_loader.hpp"
 
@@ -47,10 +51,16 @@ public:
      nb_disabled, source, empty banks); its per-call stats (last_stats /
      total_stats) count escapes and kills per rank, and events only in
      SyncCycle::events.  nb_particles_per_cycle above MAX_PARTICLES_VECTOR
-     takes the host loop (Layer::simulate then chunks each call). */
+     takes the host loop (Layer::simulate then chunks each call).
+     After the device-resident loop fails (max_cycles, event cap, tally
+     range) the Layers' banks are not brought back from the device, so
+     their state is not a consistent one: spin() and dump() then throw. */
   long long spin(long long max_cycles = -1);
 
   bool device_resident = true;
+  /* the device-resident loop runs a single rank's cycles concurrently when
+     that is exact (see mcgpu_sync_cycle); true: always one after the other */
+  bool cycle_by_cycle = false;
   /* launch shape of the device-resident loop's transport (0 = defaults) */
   mcgpu_options device_options{};
 
@@ -71,8 +81,17 @@ public:
   };
   std::vector<RankTimes> times;
   double elapsed = 0; /* seconds of the last spin() */
+  /* per rank, the statistics windows of every spin() so far (stats.csv).
+     The device-resident loop runs all ranks in one launch per cycle, so
+     every rank gets the same windows; the host loop times each rank's
+     Layer::simulate on its own and counts the rest of the cycle (the other
+     ranks' turns and the exchange, where an MPI rank would wait in
+     MPI_Sendrecv) as its send time. */
+  std::vector<std::vector<TimerWindow>> windows;
 
 private:
+  bool usable_ = true;
+  void check_usable() const;
   long long spin_host(long long max_cycles);
   long long spin_device(long long max_cycles);
 };

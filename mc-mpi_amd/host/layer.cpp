@@ -3,6 +3,7 @@
  * transport loop replaced by the GPU kernels behind include/mcgpu.h.
  */
 #include "layer.hpp"
+#include "formats.hpp"
 
 #include <math.h>
 #include <stdio.h>
@@ -223,8 +224,7 @@ std::vector<Particle> from_wire(const std::vector<WireParticle> &ws) {
 void Layer::dump_WA() {
   FILE *file = fopen("WA.out", "w");
   if (!file) throw Error(MCGPU_ERR_ARG, "Couldn't open file WA.out for writing.");
-  for (int i = 0; i < (int)weights_absorbed.size(); ++i)
-    fprintf(file, "%.4e %.3e\n", x_min + (i * dx) + 0.5 * dx, weights_absorbed[i] / dx);
+  write_wa<real_t>(file, weights_absorbed.data(), (int)weights_absorbed.size(), x_min, dx);
   fclose(file);
 }
 

@@ -7,10 +7,12 @@
  * Prints the elapsed seconds on stdout, like src/main.cpp:90-91.
  *
  *   mcgpu_run config.yaml [history|event] [device] [--ranks R] [--out DIR] [--host-loop]
+ *             [--cycle-by-cycle]
  *
  * By default the loop runs device-resident (mcgpu_sync_cycle); --host-loop
  * calls each rank's Layer::simulate in turn and exchanges on the host (the
- * kernel argument applies to that loop).
+ * kernel argument applies to that loop).  --cycle-by-cycle keeps a single
+ * rank's cycles one after the other (otherwise they run concurrently).
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -24,18 +26,20 @@ int main(int argc, char **argv) {
   if (argc < 2) {
     fprintf(stderr,
             "usage: %s config.yaml [history|event] [device] [--ranks R] [--out DIR] "
-            "[--host-loop]\n",
+            "[--host-loop] [--cycle-by-cycle]\n",
             argv[0]);
     return 2;
   }
   int ranks = 1, kernel = MCGPU_KERNEL_HISTORY, device = -1, positional = 0;
-  bool host_loop = false;
+  bool host_loop = false, cycle_by_cycle = false;
   std::string out = "out";
   for (int i = 2; i < argc; ++i) {
     if (!strcmp(argv[i], "--ranks") && i + 1 < argc) {
       ranks = atoi(argv[++i]);
     } else if (!strcmp(argv[i], "--host-loop")) {
       host_loop = true;
+    } else if (!strcmp(argv[i], "--cycle-by-cycle")) {
+      cycle_by_cycle = true;
     } else if (!strcmp(argv[i], "--out") && i + 1 < argc) {
       out = argv[++i];
     } else if (positional == 0) {
@@ -55,6 +59,7 @@ int main(int argc, char **argv) {
     mcgpu::SyncCycle run(opt);
     for (mcgpu::Layer &L : run.layers) L.options.kernel = kernel;
     run.device_resident = !host_loop;
+    run.cycle_by_cycle = cycle_by_cycle;
     run.spin();
     printf("%lf\n", run.elapsed);
     run.dump(out);

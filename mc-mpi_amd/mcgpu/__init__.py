@@ -8,9 +8,9 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (ERR_ARG, ERR_EVENT_CAP, ERR_HIP, ERR_NO_DEVICE, ERR_NOMEM, ERR_RANGE,
-                   KERNEL_EVENT, KERNEL_HISTORY, OK, PARTICLE, Bank, CycleRank, CycleStats,
-                   McgpuError, Options, Slab, Stats, lib, status_string)
+from ._lib import (CYCLE_WINDOW_FN, ERR_ARG, ERR_EVENT_CAP, ERR_HIP, ERR_NO_DEVICE, ERR_NOMEM,
+                   ERR_RANGE, KERNEL_EVENT, KERNEL_HISTORY, OK, PARTICLE, Bank, CycleRank,
+                   CycleStats, McgpuError, Options, Slab, Stats, lib, status_string)
 
 __all__ = ["Context", "Batch", "PARTICLE", "McgpuError", "KERNEL_HISTORY", "KERNEL_EVENT", "lib",
            "bank_to_device", "bank_from_device", "make_options", "options_dict", "check_tensor",
@@ -337,7 +337,8 @@ class Context:
 
     # ------------------------------------------------------------ cycle loop
     def sync_cycle(self, ranks, sigs, absorption_rates, weights_absorbed, nb_particles_per_cycle,
-                   nb_particles, max_cycles=0, options=None):
+                   nb_particles, max_cycles=0, options=None, statistics_cycle_time=None,
+                   cycle_by_cycle=False):
         """mcgpu_sync_cycle: the synchronous multi-rank loop, every rank on
         this GPU.  `ranks`: one dict per rank with min_index, max_index, dx,
         particle_min_weight, left_border, right_border, cell_offset,
@@ -345,7 +346,9 @@ class Context:
         source nb_particles_create, seed, x_ini, wmc and nb_disabled.
         sigs / absorption_rates / weights_absorbed: host float64 arrays of
         all ranks' cells; weights_absorbed is added to in place.  Returns
-        (per-rank output dicts, stats dict)."""
+        (per-rank output dicts, stats dict); with statistics_cycle_time the
+        stats dict also holds "windows", the loop's statistics windows.
+        cycle_by_cycle: never run a single rank's cycles concurrently."""
         assert weights_absorbed.dtype == np.float64 and weights_absorbed.flags.c_contiguous
         sigs = np.ascontiguousarray(sigs, np.float64)
         ar = np.ascontiguousarray(absorption_rates, np.float64)
@@ -364,6 +367,14 @@ class Context:
             k.x_ini, k.wmc = d.get("x_ini", 0.0), d.get("wmc", 0.0)
             k.nb_disabled = d.get("nb_disabled", 0)
         st = CycleStats()
+        st.cycle_by_cycle = 1 if cycle_by_cycle else 0
+        windows = []
+        if statistics_cycle_time is not None:
+            def _on_window(_user, w):
+                windows.append(w.contents.as_dict())
+            cb = CYCLE_WINDOW_FN(_on_window)
+            st.statistics_cycle_time = statistics_cycle_time
+            st.on_window = cb
         opt = options if options is not None else Options()
         rc = lib().mcgpu_sync_cycle(self._h, len(ranks), arr, sigs.ctypes.data, ar.ctypes.data,
                                     weights_absorbed.ctypes.data, len(weights_absorbed),
@@ -373,7 +384,10 @@ class Context:
         out = [{f: getattr(k, f) for f in ("nb_disabled", "n_left", "n_right", "n_killed",
                                            "n_remaining", "nb_particles_create", "seed")}
                for k in arr]
-        return out, st.as_dict()
+        d = st.as_dict()
+        if statistics_cycle_time is not None:
+            d["windows"] = windows
+        return out, d
 
     # ------------------------------------------------------------ helpers
     def rng_real_step(self, seeds, reals, stream=None):

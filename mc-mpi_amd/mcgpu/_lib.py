@@ -68,12 +68,28 @@ class CycleRank(C.Structure):
                 ("n_right", C.c_int64), ("n_killed", C.c_int64), ("n_remaining", C.c_int64)]
 
 
-class CycleStats(C.Structure):
-    _fields_ = [("cycles", C.c_int64), ("events", C.c_uint64), ("collisions", C.c_uint64),
-                ("kernel_ms", C.c_double), ("wall_ms", C.c_double)]
+class CycleWindow(C.Structure):
+    """include/mcgpu.h mcgpu_cycle_window: one statistics window of the loop."""
+    _fields_ = [("starttime", C.c_double), ("endtime", C.c_double), ("time_comp", C.c_double),
+                ("time_send", C.c_double), ("time_recv", C.c_double), ("time_idle", C.c_double),
+                ("nb_cycles", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+CYCLE_WINDOW_FN = C.CFUNCTYPE(None, C.c_void_p, C.POINTER(CycleWindow))
+
+
+class CycleStats(C.Structure):
+    _fields_ = [("cycles", C.c_int64), ("events", C.c_uint64), ("collisions", C.c_uint64),
+                ("kernel_ms", C.c_double), ("wall_ms", C.c_double),
+                ("statistics_cycle_time", C.c_double), ("on_window", CYCLE_WINDOW_FN),
+                ("window_user", C.c_void_p), ("cycle_by_cycle", C.c_int32), ("pad_", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k in ("cycles", "events", "collisions", "kernel_ms",
+                                              "wall_ms")}
 
 
 class McgpuError(RuntimeError):

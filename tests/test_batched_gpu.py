@@ -103,6 +103,42 @@ def test_batched_equals_unbatched_and_oracle(ctx, oracle):
     batch.close()
 
 
+def test_batched_layout_prefix_at_a_tenth_of_its_particles(ctx, oracle):
+    """The first 256 slabs of the 4096-slab layout (32 to 8192 cells, a tenth
+    of their particles: 3.9e6 histories): every slab's bank, exit codes and
+    tally bit-identical to its own mcgpu_simulate_device call, and six slabs
+    (the largest and smallest by cells and by particles among them) to the
+    oracle."""
+    import mcgpu
+    slabs, sigs, ar, bank = _batch_inputs(256, 0.1)
+    _say(f"layout prefix: {len(slabs)} slabs, {len(bank)} particles, {len(sigs)} cells")
+    ref_bank, ref_res, ref_w = _unbatched(ctx, slabs, sigs, ar, bank)
+    d_sigs, d_ar, d_w, d_bank, d_res = _device(slabs, sigs, ar, bank)
+    batch = mcgpu.Batch(ctx, slabs, d_sigs, d_ar, d_w, d_bank, d_res)
+    st = batch.run()
+    batch.close()
+    got = mcgpu.bank_from_device(d_bank)
+    res = d_res.cpu().numpy()
+    w = d_w.cpu().numpy()
+    assert _same_bank(got, ref_bank)
+    assert np.array_equal(res, ref_res)
+    assert np.array_equal(w.view(np.uint64), ref_w.view(np.uint64))
+    assert st["n_left"] + st["n_right"] + st["n_killed"] == len(bank)
+    m = np.array([s[1] - s[0] for s in slabs])
+    n = np.array([s[6] for s in slabs])
+    pick = {int(np.argmax(m)), int(np.argmin(m)), int(np.argmin(n))}
+    pick |= set(int(i) for i in np.argsort(n)[-3:] if m[i] < 3000)  # large banks, short slabs
+    for s in sorted(pick):
+        lo, hi, dx, mw, coff, poff, k = slabs[s]
+        _say(f"  oracle: slab {s}, {hi - lo} cells, {k} particles")
+        o = oracle.simulate_bank(64, lo, hi - lo, dx, mw, sigs[coff:coff + hi - lo],
+                                 ar[coff:coff + hi - lo], bank[poff:poff + k], nthread=16)
+        assert _same_bank(got[poff:poff + k], o["bank"]), s
+        assert np.array_equal(res[poff:poff + k].astype(np.int32), o["result"]), s
+        assert np.array_equal(w[coff:coff + hi - lo].view(np.uint64),
+                              o["tally_exact"].view(np.uint64)), s
+
+
 def test_graph_replays_match_eager(ctx):
     """Replay the captured graph three times, restoring the inputs with eager
     torch copies and synchronizing between replays; every replay must equal

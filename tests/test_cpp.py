@@ -111,6 +111,73 @@ def test_mcgpu_run_five_ranks_on_one_gpu(tmp_path):
                                        9.999999960041972003e-13, 500, nthread=8, tally_exact=True)
     want = np.concatenate([L.weights_absorbed for L in layers]) / layers[0].dx
     np.testing.assert_array_equal(w[:, 2], want)
-    stats = (tmp_path / "out" / "stats.csv").read_text().splitlines()
-    assert stats[0].startswith("rank, starttime, endtime, time_comp") and len(stats) == 6
-    assert all(line.endswith(f"{cycles}, ") for line in stats[1:])
+    _check_stats(tmp_path / "out" / "stats.csv", 5, cycles)
+
+
+def _check_stats(path, ranks, cycles, period=0.1):
+    """out/stats.csv of a run: the reference's header, then per rank its
+    statistics windows (src/worker_sync.cpp:25-27,121-133): contiguous in
+    time, every window but the last longer than the statistics period, the
+    cycles counted adding up to all cycles but the one that ended the loop,
+    and the phase times adding up to the window's length."""
+    lines = path.read_text().splitlines()
+    assert lines[0] == ("rank, starttime, endtime, time_comp, time_send, time_recv, "
+                        "time_idle, nb_cycles, ")
+    rows = {r: [] for r in range(ranks)}
+    for ln in lines[1:]:
+        assert ln.endswith(", ")
+        f = [v.strip() for v in ln.split(",")[:-1]]
+        assert len(f) == 8
+        rows[int(f[0])].append([float(v) for v in f[1:7]] + [int(f[7])])
+    for r in range(ranks):
+        w = rows[r]
+        assert w and sum(x[6] for x in w) == cycles - 1, (r, w)
+        for a, b in zip(w, w[1:]):
+            assert a[1] == b[0]                      # contiguous
+            assert a[1] - a[0] > period              # closed after the period
+        for x in w:
+            assert x[1] >= x[0] and min(x[2:6]) >= 0
+            assert abs(sum(x[2:6]) - (x[1] - x[0])) <= 1e-3 + 1e-2 * (x[1] - x[0])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["r1_c100", "r2_c1000", "r3_c200_lazy", "r4_c300_s7",
+                                  "r5_c1000"])
+def test_mcgpu_run_on_reference_fixture_configs(tmp_path, case):
+    """`mcgpu_run config.yaml --ranks R` on the configs the reference's own
+    program ran for tests/golden/sync/: out/config.yaml equals the
+    reference's byte for byte (hostname aside); out/weights.csv equals the
+    FP64 oracle's loop (the FP32 oracle's loop equals the reference program,
+    tests/test_sync_reference.py), with the oracle's number of cycles;
+    out/stats.csv has the reference's layout and window semantics."""
+    import json
+    import shutil
+    import socket
+    import numpy as np
+    import pyoracle
+    g = os.path.join(ROOT, "tests", "golden", "sync", case)
+    summary = json.load(open(os.path.join(g, "summary.json")))
+    R = summary["ranks"]
+    shutil.copy(os.path.join(g, "config.yaml"), tmp_path / "config.yaml")
+    out = _run([os.path.join(ROOT, "mc-mpi_amd", "mcgpu_run"), "config.yaml", "--ranks", str(R)],
+               str(tmp_path))
+    assert out.returncode == 0
+    fields = dict(kv.split(" ", 1) for kv in
+                  [ln for ln in out.stderr.splitlines() if ln.startswith("events ")][-1].split(", "))
+    cfg = {k.strip(): v.strip() for k, v in
+           (ln.split(":", 1) for ln in open(os.path.join(g, "config.yaml")))}
+    got = (tmp_path / "out" / "config.yaml").read_text().replace(
+        f"hostname: {socket.gethostname()}\n", "hostname: HOSTNAME\n")
+    assert got == open(os.path.join(g, "out_config.yaml")).read()
+    layers, cycles = pyoracle.sync_run(64, float(cfg["x_min"]), float(cfg["x_max"]),
+                                       float(cfg["x_ini"]), R, int(cfg["nb_cells"]),
+                                       int(cfg["nb_particles"]),
+                                       float(cfg["particle_min_weight"]),
+                                       int(cfg["nb_particles_per_cycle"]), nthread=8,
+                                       tally_exact=True)
+    assert int(fields["cycles"]) == cycles
+    assert int(fields["disabled"]) == int(cfg["nb_particles"])
+    w = np.loadtxt(tmp_path / "out" / "weights.csv", delimiter=",", skiprows=1)
+    want = np.concatenate([L.weights_absorbed for L in layers]) / layers[0].dx
+    np.testing.assert_array_equal(w[:, 2], want)
+    _check_stats(tmp_path / "out" / "stats.csv", R, cycles)

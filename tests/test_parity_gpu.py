@@ -140,7 +140,9 @@ def test_device_create_particles(ctx, oracle):
 
 # ------------------------------------------------------------- transport
 
-CASES = [("slab100", 20_000), ("layer_rank2of5", 20_000), ("adjoint1000", 3_000), ("slab10k", 400)]
+# the 100-cell slab at its full 1e5 particles; the others at prefixes of
+# their banks (full sizes below, through size-independent properties)
+CASES = [("slab100", 100_000), ("layer_rank2of5", 20_000), ("adjoint1000", 3_000), ("slab10k", 400)]
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
@@ -347,6 +349,58 @@ def test_repeated_calls_accumulate(ctx, oracle):
     _assert_bank_equal(b1, o1["bank"])
     _assert_bank_equal(b2, o2["bank"])
     assert np.array_equal(w, o1["tally_exact"] + o2["tally_exact"])
+
+
+# ---------------------------------------------------- the reference's GPU recipe
+
+_RECIPE = {}
+
+
+def _reference_recipe(oracle):
+    """The recipe's layer, bank and oracle run (computed once for both kernels)."""
+    if not _RECIPE:
+        import mcgpu
+        x_ini = np.sqrt(2.0) / 2.0
+        L = oracle.OracleLayer.decompose_domain(64, 0.0, 1.0, x_ini, 1, 0, 1000, 1_000_000, 0.0)
+        # copies: the layer's arrays are views of memory freed with the layer
+        bank0 = np.array(L.particles, dtype=mcgpu.PARTICLE)
+        sigs, ar = np.array(L.sigs), np.array(L.absorption_rates)
+        o = oracle.simulate_bank(64, 0, 1000, L.dx, 0.0, sigs, ar, bank0, nthread=16)
+        _RECIPE.update(bank0=bank0, sigs=sigs, ar=ar, dx=L.dx, o=o)
+    return _RECIPE
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_reference_gpu_recipe_at_its_size(ctx, oracle, kernel):
+    """The reference's own CPU-against-GPU recipe (src/test_culayer.cu:27-87):
+    decompose_domain(0, 1, sqrt(2)/2, 1 rank, 1000 cells, 1e6 particles,
+    minimum weight 0) -- the created source (seed chain from 5127801,
+    isotropic mu, weight 1e-6) -- transported to the end.  The reference
+    compares the tallies to 1e-4; here every history's final state, exit
+    code, event and collision counts and the exact tally against the FP64
+    oracle, bit for bit."""
+    import torch
+    import mcgpu
+    _say(f"reference GPU recipe: 1e6 particles, kernel {kernel}")
+    rc = _reference_recipe(oracle)
+    bank0, sigs, ar, dx, o = rc["bank0"], rc["sigs"], rc["ar"], rc["dx"], rc["o"]
+    n = len(bank0)
+    assert n == 1_000_000 and np.all(bank0["wmc"] == 1e-6)
+    bank = mcgpu.bank_to_device(bank0)
+    w = torch.zeros(1000, dtype=torch.float64, device="cuda")
+    res = torch.empty(n, dtype=torch.int8, device="cuda")
+    ev = torch.empty(n, dtype=torch.int64, device="cuda")
+    co = torch.empty(n, dtype=torch.int64, device="cuda")
+    st, _ = ctx.simulate_device(bank, torch.from_numpy(sigs).cuda(), torch.from_numpy(ar).cuda(),
+                                w, 0, 1000, dx, 0.0, result=res, events=ev, collisions=co,
+                                options=mcgpu.make_options(kernel))
+    _say(f"  events={st['events']} kernel_ms={st['kernel_ms']:.1f}")
+    _assert_bank_equal(mcgpu.bank_from_device(bank), o["bank"])
+    assert np.array_equal(res.cpu().numpy().astype(np.int32), o["result"])
+    assert np.array_equal(ev.cpu().numpy().view(np.uint64), o["events"])
+    assert np.array_equal(co.cpu().numpy().view(np.uint64), o["collisions"])
+    assert st["n_killed"] == 0          # minimum weight 0: every history escapes
+    _assert_tally(w.cpu().numpy(), o)
 
 
 # ---------------------------------------------------- full-size properties
