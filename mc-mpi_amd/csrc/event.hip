@@ -100,60 +100,185 @@ __device__ __forceinline__ void put(const pool_ref &h, int j, const mc_state &p,
   if (COLL) st_nt(h.col() + j, col);
 }
 
-/* An in-place pass (no compaction) writes back only what the event changed:
-   seed and wmc always; the cell index after a crossing; x and mu after a
-   collision; the id when the on-edge flag changed.  (t0 stays.) */
-template <bool COLL>
-__device__ __forceinline__ void put_changed(const pool_ref &h, int j, const mc_state &p,
-                                            uint32_t id, uint32_t col, bool on_edge,
-                                            bool was_on_edge) {
-  st_nt(h.seed() + j, p.seed);
-  st_nt(h.wmc() + j, p.wmc);
-  if (COLL) st_nt(h.col() + j, col);
-  if (on_edge) {
-    st_nt(h.index() + j, p.index);
+/* ------------------------------------------------------------ the pipeline
+   A pass gives every live slot of the wave's pool one event, 64 slots (a
+   chunk) at a time.  Vector memory operations of a wave complete in the
+   order they were issued, so a wait for one load is a wait for every
+   earlier one: a chunk that loads its state, then its cell, then its exp
+   table row, waiting for each, pays three memory latencies in a row.  Here
+   a chunk's event runs in four stages, in four successive iterations of the
+   wave's loop; each iteration advances four chunks by one stage, the oldest
+   first:
+     D  the exp table row has arrived: exp, dw, wmc, the tally (LDS), the
+        exit of a finished history (to its bank slot), the write-back of the
+        slot (in place, or compacted into the other half);
+     C  the state and the cell have arrived: the draw, the edge distance,
+        the collision decision, the new x and mu; load the exp table row;
+     B  id, index and the event-count origin have arrived: load the rest of
+        the state (x only where it is not implicit) and the cell;
+     A  load id, index and the event-count origin of the next chunk.
+   Every wait of an iteration is then for operations issued in the
+   iteration before.  Pool accesses are raw buffer loads and stores whose
+   masked lanes take an offset beyond the pool (no memory access; loads give
+   0), and cell and table loads take a valid address on every lane, so each
+   stage issues the same memory instructions on every path: the compiler's
+   wait counts stay exact.  Passes between compactions update the slots in
+   place, so their chunks stream through the pipeline without a break (a
+   slot is written in stage D of one pass before stage A of the next pass
+   reads it, given at least four chunks); the pipeline drains at each
+   compaction and before the pool is refilled. */
+
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+constexpr int OOB = 0x7FFFFFF0; /* a byte offset beyond every pool half */
+
+__device__ __forceinline__ rsrc_t half_rsrc(unsigned char *half, int P) {
+  return __builtin_amdgcn_make_buffer_rsrc(half, (short)0, (int)(SLOT_BYTES * (size_t)P),
+                                           0x00020000);
+}
+__device__ __forceinline__ uint64_t ld64(rsrc_t r, int voff, int soff) {
+  const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+  return ((uint64_t)v.y << 32) | v.x;
+}
+__device__ __forceinline__ uint32_t ld32(rsrc_t r, int voff, int soff) {
+  return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+}
+__device__ __forceinline__ void st64(rsrc_t r, int voff, int soff, uint64_t v) {
+  u32x2 w;
+  w.x = (uint32_t)v;
+  w.y = (uint32_t)(v >> 32);
+  __builtin_amdgcn_raw_buffer_store_b64(w, r, voff, soff, 0);
+}
+__device__ __forceinline__ void st32(rsrc_t r, int voff, int soff, uint32_t v) {
+  __builtin_amdgcn_raw_buffer_store_b32(v, r, voff, soff, 0);
+}
+__device__ __forceinline__ double as_f64(uint64_t v) { return __builtin_bit_cast(double, v); }
+__device__ __forceinline__ uint64_t as_u64(double v) { return __builtin_bit_cast(uint64_t, v); }
+
+/* A copy the compiler cannot merge with its source: a stage copies the
+   loads it consumes, so the next load into the same variable needs no copy
+   (and no wait) at the loop's back edge. */
+__device__ __forceinline__ uint64_t opq64(uint64_t v) {
+  uint64_t r;
+  asm volatile("v_mov_b64 %0, %1" : "=v"(r) : "v"(v));
+  return r;
+}
+__device__ __forceinline__ uint32_t opq32(uint32_t v) {
+  uint32_t r;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(r) : "v"(v));
+  return r;
+}
+
+/* A kernel argument as a scalar register value of its own.  The argument
+   block arrives in wide scalar loads, and a field the loop uses would
+   otherwise keep the whole block live, or reload all of it from its spill
+   lanes at every use. */
+template <typename T> __device__ __forceinline__ T own_sreg(T v) {
+  asm volatile("" : "+s"(v));
+  return v;
+}
+
+/* The call's arguments as a block's LDS copy: the refill and the exit of a
+   finished history (a few per ten thousand events) read the bank pointers
+   from there when they run, behind a compiler barrier that keeps the loads
+   in place (not hoisted out of the loop), so the pointers hold no registers
+   during the passes. */
+typedef mc_transport_args sargs;
+__device__ __forceinline__ const sargs &fresh(const sargs &v) {
+  asm volatile("" ::: "memory");
+  return v;
+}
+
+/* A pointer the compiler can no longer trace to a kernel argument (read from
+   LDS, or passed through own_sreg) would be accessed with FLAT instructions;
+   a FLAT access still in flight makes every later wait a wait for all
+   memory operations, which would undo the pipeline.  Pointers to device
+   memory are therefore retyped as global ones before use. */
+#define MC_GLOBAL __attribute__((address_space(1)))
+template <typename T> __device__ __forceinline__ MC_GLOBAL T *gbl(T *p) {
+  return (MC_GLOBAL T *)p;
+}
+__device__ __forceinline__ mc_cell ld_cell(MC_GLOBAL const mc_cell *p) {
+  mc_cell c;
+  c.sig_a = p->sig_a;
+  c.sig_i = p->sig_i;
+  return c;
+}
+
+__device__ __forceinline__ void refill_state(const sargs &v, int64_t i, mc_state &p) {
+  const mcgpu_particle *aos = v.aos;
+  if (aos) {
+    MC_GLOBAL const mcgpu_particle *q = gbl(aos) + i;
+    p.seed = q->seed;
+    p.x = q->x;
+    p.mu = q->mu;
+    p.wmc = q->wmc;
+    p.index = q->index;
   } else {
-    st_nt(h.x() + j, p.x);
-    st_nt(h.mu() + j, p.mu);
-  }
-  if (on_edge != was_on_edge) st_nt(h.id() + j, on_edge ? (id | ON_EDGE) : id);
-}
-
-/* the registers of one pool slot */
-struct slot_regs {
-  mc_state p;
-  uint32_t id, ev, col;
-  bool on_edge; /* the slot's x was implicit */
-  bool valid;
-};
-
-template <bool COLL>
-__device__ __forceinline__ void load_slot(const pool_ref &h, int j, double dx, uint32_t pass,
-                                          slot_regs &r) {
-  r.id = ld_nt(h.id() + j);
-  r.p.seed = ld_nt(h.seed() + j);
-  r.p.mu = ld_nt(h.mu() + j);
-  r.p.wmc = ld_nt(h.wmc() + j);
-  r.p.index = ld_nt(h.index() + j);
-  r.ev = pass - ld_nt(h.t0() + j);
-  r.col = COLL ? ld_nt(h.col() + j) : 0u;
-  r.on_edge = r.id != DEAD_ID && (r.id & ON_EDGE);
-  if (r.on_edge) {
-    r.id &= ~ON_EDGE;
-    r.p.x = (r.p.mu < 0 ? r.p.index + 1 : r.p.index) * dx;
-  } else if (r.id != DEAD_ID) {
-    r.p.x = ld_nt(h.x() + j);
+    p.seed = gbl(v.soa.seed)[i];
+    p.x = gbl(v.soa.x)[i];
+    p.mu = gbl(v.soa.mu)[i];
+    p.wmc = gbl(v.soa.wmc)[i];
+    p.index = gbl(v.soa.index)[i];
   }
 }
+
+/* the same stores as store_state, then the per-history counts asked for */
+__device__ __forceinline__ void exit_state(const sargs &v, int64_t i, const mc_state &p, int code,
+                                           uint32_t ev, uint32_t col) {
+  mcgpu_particle *aos = v.aos;
+  if (aos) {
+    MC_GLOBAL mcgpu_particle *q = gbl(aos) + i;
+    q->seed = p.seed;
+    q->x = p.x;
+    q->mu = p.mu;
+    q->wmc = p.wmc;
+    q->index = p.index;
+    int32_t *r = v.result32;
+    if (r) gbl(r)[i] = code;
+  } else {
+    gbl(v.soa.seed)[i] = p.seed;
+    gbl(v.soa.x)[i] = p.x;
+    gbl(v.soa.mu)[i] = p.mu;
+    gbl(v.soa.wmc)[i] = p.wmc;
+    gbl(v.soa.index)[i] = p.index;
+    int8_t *r = v.result8;
+    if (r) gbl(r)[i] = (int8_t)code;
+  }
+  uint64_t *e = v.events;
+  if (e) gbl(e)[i] = ev;
+  uint64_t *c = v.collisions;
+  if (c) gbl(c)[i] = col;
+}
+
+/* Threads per block at most: 1024, 16 waves per CU when the block's tally
+   fills the LDS.  The four stages keep about 60 values per lane live across
+   the calls of the rare math paths, which leave only the callee-saved
+   registers intact (8 of every 16 from v40 up); the kernel fits in 128
+   registers without spills as long as every wave-uniform value (the pool
+   size, the pass and chunk counters, the pool's buffer resource) stays
+   scalar -- a pool size taken as divergent puts the buffer resource in
+   vector registers and every pool access in a loop over its lanes. */
+constexpr int EVENT_MAX_BLOCK = 1024;
 
 template <bool LDS_TALLY, bool LDS_CELLS, bool COLL>
-__global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned char *pools, int P,
-                                                int compact_every) {
+__global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
+                                                           unsigned char *pools, int P,
+                                                           int compact_every) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const mc_coef K = mc_coef_live();
+  /* coefficients as constants: scalar moves at their uses, no registers */
+  const mc_coef K = mc_coef_default();
   /* block totals: events, collisions, left, right, killed */
   __shared__ unsigned long long s_tot[5];
-  const int m = a.L.m;
+  __shared__ mc_transport_args s_a;
+  const sargs &v = s_a;
+  if (threadIdx.x == 0) s_a = a;
+  mc_layer_k L;
+  L.index_start = own_sreg(a.L.index_start);
+  L.m = own_sreg(a.L.m);
+  L.dx = own_sreg(a.L.dx);
+  L.min_weight = own_sreg(a.L.min_weight);
+  const int m = L.m;
   unsigned long long *t_lo, *t_hi;
   size_t off = 0;
   if (LDS_TALLY) {
@@ -168,12 +293,11 @@ __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned ch
     t_lo = a.fx;
     t_hi = a.fx + m;
   }
-  const mc_cell *cells = a.cells;
-  if (LDS_CELLS) {
-    mc_cell *lc = reinterpret_cast<mc_cell *>(smem + off);
-    for (int c = threadIdx.x; c < m; c += blockDim.x) lc[c] = a.cells[c];
-    cells = lc;
-  }
+  /* the cells in LDS when they fit beside the tally, else in device memory */
+  MC_GLOBAL const mc_cell *cells_g = gbl(own_sreg(a.cells));
+  mc_cell *cells_l = reinterpret_cast<mc_cell *>(smem + off);
+  if (LDS_CELLS)
+    for (int c = threadIdx.x; c < m; c += blockDim.x) cells_l[c] = ld_cell(cells_g + c);
   if (threadIdx.x < 5) s_tot[threadIdx.x] = 0;
   __syncthreads();
 
@@ -184,30 +308,42 @@ __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned ch
       (int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
   const size_t half_bytes = SLOT_BYTES * (size_t)P;
   unsigned char *mine = pools + (size_t)wave * 2 * half_bytes;
-  const uint32_t max_ev = a.max_events > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)a.max_events;
+  const uint32_t max_ev =
+      own_sreg(a.max_events > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)a.max_events);
+  const double dx = L.dx;
+  /* field offsets in a half (bytes) */
+  const int F_X = 8 * P, F_MU = 16 * P, F_WMC = 24 * P, F_INDEX = 32 * P, F_ID = 36 * P,
+            F_T0 = 40 * P, F_COL = 44 * P;
+  const unsigned ce = (unsigned)compact_every;
 
   int cur = 0;
   int n_pool = 0; /* slots in use in the current half (wave-uniform) */
   bool queue_done = false;
   bool capped = false;
 
-  for (unsigned pass = 0;; ++pass) {
+  for (unsigned pass = 0;;) {
     /* ---- refill from the work queue (wave-uniform condition) */
     if (!queue_done && n_pool <= P - 64) {
       const int want = P - n_pool;
+      const sargs &w = fresh(v);
       unsigned long long base = 0;
-      if (lane == 0) base = atomicAdd(&a.ctr->next, (unsigned long long)want);
+      if (lane == 0)
+        base = __hip_atomic_fetch_add(&gbl(w.ctr)->next, (unsigned long long)want,
+                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       base = mc_readlane64(base, 0);
+      /* a load from LDS counts as divergent: made uniform again, so that
+         the pool size and everything derived from it stay scalar */
+      const unsigned long long n_bank = mc_readlane64((unsigned long long)(int64_t)w.n, 0);
       int take = 0;
-      if (base < (unsigned long long)a.n) {
-        const unsigned long long left = (unsigned long long)a.n - base;
+      if (base < n_bank) {
+        const unsigned long long left = n_bank - base;
         take = left < (unsigned long long)want ? (int)left : want;
       }
       if (take < want) queue_done = true;
       const pool_ref h{mine + cur * half_bytes, P};
       for (int k = lane; k < take; k += 64) {
         mc_state p;
-        load_state(a, (int64_t)(base + k), p);
+        refill_state(fresh(v), (int64_t)(base + k), p);
         put<COLL>(h, n_pool + k, p, (uint32_t)(base + k), (uint32_t)pass, 0u, false);
       }
       n_pool += take;
@@ -216,101 +352,248 @@ __global__ __launch_bounds__(1024) void k_event(mc_transport_args a, unsigned ch
     }
     if (n_pool == 0) {
       if (queue_done) break;
+      ++pass;
       continue;
     }
 
-    /* ---- one event per live particle */
-    const bool compact = (pass % (unsigned)compact_every) == 0;
-    const pool_ref src{mine + cur * half_bytes, P};
-    const pool_ref dst{mine + (cur ^ 1) * half_bytes, P};
+    /* ---- a segment: the passes up to the next compaction (one pass when
+       the pool holds fewer than four chunks), streamed through the stages */
+    const int nch = (n_pool + 63) >> 6;
+    const unsigned last = nch >= 4 ? pass + (ce - pass % ce) % ce : pass;
+    const int total = (int)(last - pass + 1) * nch;
+    const bool ends_compacting = last % ce == 0;
+    const rsrc_t src = half_rsrc(mine + cur * half_bytes, P);
+    const rsrc_t dst = half_rsrc(mine + (cur ^ 1) * half_bytes, P);
     int wpos = 0; /* wave-uniform output position of the compaction */
-    unsigned int pass_ev = 0, pass_col = 0; /* this lane's events of the pass */
-    /* software pipeline: the next slot is loaded while this one's event is
-       computed, so its memory latency overlaps the FP64 work */
-    slot_regs nxt;
-    nxt.valid = lane < n_pool;
-    if (nxt.valid) load_slot<COLL>(src, lane, a.L.dx, pass, nxt);
-    for (int j0 = 0; j0 < n_pool; j0 += 64) { /* wave-uniform trip count */
-      const int j = j0 + lane;
-      mc_state p = nxt.p;
-      uint32_t id = nxt.valid ? nxt.id : DEAD_ID, ev = nxt.ev, col = nxt.col;
-      const bool was_on_edge = nxt.valid && nxt.on_edge;
-      if (j0 + 64 < n_pool) {
-        nxt.valid = j + 64 < n_pool;
-        if (nxt.valid) load_slot<COLL>(src, j + 64, a.L.dx, pass, nxt);
-      }
-      const bool live = id != DEAD_ID;
-      bool keep = false;
-      bool on_edge = false; /* set by a crossing */
-      if (live) {
-        if (mc_alive(p, a.L) && ev < max_ev) {
-          const int c = p.index - a.L.index_start;
-          bool coll;
-          double di;
-          const double dw = mc_event(p, cells, c, a.L.dx, K, coll, di);
-          on_edge = !coll;
-          ++ev;
-          ++pass_ev;
-          col += coll ? 1u : 0u;
-          pass_col += coll ? 1u : 0u;
-          mc_tally(t_lo, t_hi, a.flags, &a.ctr->status, c, dw);
-        }
-        keep = mc_alive(p, a.L) && ev < max_ev;
-        if (!keep) { /* history finished: write it to its bank slot */
-          int code = mc_exit_code(p, a.L);
-          if (mc_alive(p, a.L)) {
-            capped = true;
-            code = 2;
+    int a_j = 0;           /* stage A's next chunk: index in the pool */
+    unsigned a_pass = pass; /* and pass */
+    unsigned int lane_ev = 0, lane_col = 0;
+
+    /* stage registers: A -> B */
+    uint32_t a_id = 0, a_t0 = 0;
+    int32_t a_index = 0;
+    bool a_valid = false;
+    int a_slot = 0;
+    unsigned a_q = 0;
+    /* B -> C */
+    uint64_t b_seed = 0, b_mu = 0, b_wmc = 0, b_x = 0;
+    mc_cell b_cell;
+    b_cell.sig_a = 0.0;
+    b_cell.sig_i = 0.0;
+    uint32_t b_id = 0, b_ev = 0, b_col = 0;
+    int32_t b_index = 0;
+    bool b_live = false, b_on_edge = false;
+    int b_slot = 0;
+    unsigned b_q = 0;
+    /* C -> D */
+    mc_state c_p;
+    c_p.seed = 0;
+    c_p.x = 0.0;
+    c_p.mu = 0.0;
+    c_p.wmc = 0.0;
+    c_p.index = 0;
+    uint32_t c_id = 0, c_ev = 0, c_col = 0;
+    int32_t c_index_new = 0;
+    int c_cell = 0;
+    bool c_live = false, c_event = false, c_coll = false, c_was_on_edge = false;
+    double c_arg = 0.0;
+    mc_exp_row c_row;
+    c_row.th = 0.0;
+    c_row.tl = 0.0;
+    int c_slot = 0;
+    unsigned c_q = 0;
+
+    for (int t = 0; t < total + 3; ++t) { /* wave-uniform trip count */
+      /* ======== D: chunk t-3 */
+      {
+        /* the pass of this chunk compacts: the segment's last pass, when
+           that is a compaction pass (passes before it are not) */
+        const bool compact = ends_compacting && c_q == last;
+        bool keep = false, on_edge = false;
+        if (c_live) {
+          if (c_event) {
+            const double e = mc_exp_k_row(c_arg, &c_row, &K);
+            const double dw = (1 - e) * c_p.wmc;
+            c_p.wmc -= dw;
+            c_p.index = c_index_new;
+            on_edge = !c_coll;
+            ++c_ev;
+            ++lane_ev;
+            c_col += c_coll ? 1u : 0u;
+            lane_col += c_coll ? 1u : 0u;
+            mc_tally_with(t_lo, t_hi, c_cell, dw, [&](uint32_t g) {
+              const sargs &w = fresh(v);
+              __hip_atomic_fetch_or(gbl(w.flags) + c_cell, g, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+              if (g & 8u)
+                __hip_atomic_fetch_or(&gbl(w.ctr)->status, 2u, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+            });
           }
-          store_state(a, (int64_t)id, p, code);
-          if (a.events) a.events[id] = ev;
-          if (a.collisions) a.collisions[id] = col;
-          atomicAdd(&s_tot[code == -1 ? 2 : (code == 1 ? 3 : 4)], code == 2 ? 0ull : 1ull);
+          keep = mc_alive(c_p, L) && c_ev < max_ev;
+          if (!keep) { /* history finished: write it to its bank slot */
+            int code = mc_exit_code(c_p, L);
+            if (mc_alive(c_p, L)) {
+              capped = true;
+              code = 2;
+            }
+            exit_state(fresh(v), (int64_t)c_id, c_p, code, c_ev, c_col);
+            atomicAdd(&s_tot[code == -1 ? 2 : (code == 1 ? 3 : 4)], code == 2 ? 0ull : 1ull);
+          }
         }
+        /* the write-back: the offsets per path, then the same eight stores
+           on every path (a store on one path only would make the compiler
+           assume fewer operations in flight after the branches and wait
+           longer at the next stage); masked lanes aimed beyond the pool */
+        const uint32_t id_val = keep ? (on_edge ? (c_id | ON_EDGE) : c_id) : DEAD_ID;
+        const bool w_x = keep && !on_edge; /* x is stored unless implicit */
+        int o_seed, o_x, o_mu, o_index, o_id, o_t0, o_col;
+        if (compact) { /* wave-uniform: survivors packed into the other half */
+          const unsigned long long km = __ballot(keep);
+          const int pos = wpos + mc_mbcnt64(km);
+          wpos += __popcll(km);
+          o_seed = keep ? 8 * pos : OOB;
+          o_x = w_x ? 8 * pos : OOB;
+          o_mu = o_seed;
+          o_index = o_id = o_t0 = o_col = keep ? 4 * pos : OOB;
+        } else { /* in place: only what the event changed */
+          o_seed = keep ? 8 * c_slot : OOB;
+          o_x = o_mu = w_x ? 8 * c_slot : OOB; /* a collision's x and mu */
+          o_index = (keep && on_edge) ? 4 * c_slot : OOB;
+          o_id = (c_live && (!keep || on_edge != c_was_on_edge)) ? 4 * c_slot : OOB;
+          o_t0 = OOB; /* t0 stays */
+          o_col = keep ? 4 * c_slot : OOB;
+        }
+        const rsrc_t r = compact ? dst : src;
+        st64(r, o_seed, 0, c_p.seed);
+        st64(r, o_x, F_X, as_u64(c_p.x));
+        st64(r, o_mu, F_MU, as_u64(c_p.mu));
+        st64(r, o_seed, F_WMC, as_u64(c_p.wmc));
+        st32(r, o_index, F_INDEX, (uint32_t)c_p.index);
+        st32(r, o_id, F_ID, id_val);
+        st32(r, o_t0, F_T0, c_q + 1u - c_ev);
+        if (COLL) st32(r, o_col, F_COL, c_col);
       }
-      if (compact) {
-        /* stream compaction: survivors keep their order, ranks from the ballot */
-        const unsigned long long mask = __ballot(keep);
-        if (keep) put<COLL>(dst, wpos + mc_mbcnt64(mask), p, id, pass + 1u - ev, col, on_edge);
-        wpos += __popcll(mask);
-      } else if (live) {
-        if (keep)
-          put_changed<COLL>(src, j, p, id, col, on_edge, was_on_edge);
+      /* the stages in this order: each reads its inputs before the next one
+         overwrites them, so a load's destination is never copied while the
+         load is in flight (which would force a wait) */
+      __builtin_amdgcn_sched_barrier(0);
+      /* ======== C: chunk t-2 */
+      {
+        mc_state p;
+        p.seed = opq64(b_seed);
+        p.mu = as_f64(opq64(b_mu));
+        p.wmc = as_f64(opq64(b_wmc));
+        p.index = b_index;
+        const double x_ld = as_f64(opq64(b_x));
+        p.x = b_on_edge ? (p.mu < 0 ? b_index + 1 : b_index) * dx : x_ld;
+        mc_cell cell;
+        cell.sig_a = as_f64(opq64(as_u64(b_cell.sig_a)));
+        cell.sig_i = as_f64(opq64(as_u64(b_cell.sig_i)));
+        const uint32_t col0 = COLL ? opq32(b_col) : 0u;
+        bool event = false, coll = false;
+        int index_new = b_index;
+        double arg = 0.0;
+        const int c = b_index - L.index_start;
+        if (b_live && mc_alive(p, L) && b_ev < max_ev) {
+          event = true;
+          arg = mc_event_decide(p, cell, dx, K, coll, index_new);
+        }
+        c_row = mc_exp_lookup(arg); /* every lane: a row of the table */
+        c_p = p;
+        c_id = b_id;
+        c_ev = b_ev;
+        c_col = col0;
+        c_index_new = index_new;
+        c_cell = c;
+        c_live = b_live;
+        c_event = event;
+        c_coll = coll;
+        c_was_on_edge = b_on_edge;
+        c_arg = arg;
+        c_slot = b_slot;
+        c_q = b_q;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      /* ======== B: chunk t-1 */
+      {
+        const bool live = a_valid && a_id != DEAD_ID;
+        const bool on_edge = live && (a_id & ON_EDGE) != 0;
+        const int o8 = live ? 8 * a_slot : OOB;
+        b_seed = ld64(src, o8, 0);
+        b_mu = ld64(src, o8, F_MU);
+        b_wmc = ld64(src, o8, F_WMC);
+        b_x = ld64(src, (live && !on_edge) ? 8 * a_slot : OOB, F_X);
+        if (COLL) b_col = ld32(src, live ? 4 * a_slot : OOB, F_COL);
+        /* the cell, at an index inside the layer on every lane */
+        const unsigned cu = (unsigned)(a_index - L.index_start);
+        const unsigned ci = cu < (unsigned)m ? cu : 0u;
+        if (LDS_CELLS)
+          b_cell = cells_l[ci];
         else
-          st_nt(src.id() + j, DEAD_ID);
+          b_cell = ld_cell(cells_g + ci);
+        b_id = a_id & ~ON_EDGE;
+        b_index = (int32_t)opq32((uint32_t)a_index);
+        b_ev = a_q - a_t0;
+        asm volatile("" : "+v"(b_id), "+v"(b_ev));
+        b_live = live;
+        b_on_edge = on_edge;
+        b_slot = a_slot;
+        b_q = a_q;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      /* ======== A: chunk t */
+      {
+        const bool active = t < total; /* wave-uniform */
+        const int slot = a_j * 64 + lane;
+        a_valid = active && slot < n_pool;
+        const int o4 = a_valid ? 4 * slot : OOB;
+        a_id = ld32(src, o4, F_ID);
+        a_index = (int32_t)ld32(src, o4, F_INDEX);
+        a_t0 = ld32(src, o4, F_T0);
+        a_slot = slot;
+        a_q = a_pass;
+        if (++a_j == nch) { /* chunk index and pass of the next chunk */
+          a_j = 0;
+          ++a_pass;
+        }
       }
     }
     {
-      const unsigned long long e = mc_wave_sum(pass_ev), cl = mc_wave_sum(pass_col);
+      const unsigned long long e = mc_wave_sum(lane_ev), cl = mc_wave_sum(lane_col);
       if (lane == 0) {
         atomicAdd(&s_tot[0], e);
         atomicAdd(&s_tot[1], cl);
       }
     }
-    if (compact) {
+    if (ends_compacting) {
       n_pool = wpos;
       cur ^= 1;
-      /* compacted slots move between lanes: complete the stores first */
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     }
+    /* compacted slots move between lanes, and the next refill writes slots
+       another lane reads: complete the stores first */
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    pass = last + 1;
   }
 
-  if (capped) atomicOr(&a.ctr->status, 1u);
+  mc_counters *ctr = a.ctr;
+  if (capped) atomicOr(&ctr->status, 1u);
 
   __syncthreads();
   if (threadIdx.x == 0) {
-    atomicAdd(&a.ctr->events, s_tot[0]);
-    atomicAdd(&a.ctr->collisions, s_tot[1]);
-    atomicAdd(&a.ctr->n_left, s_tot[2]);
-    atomicAdd(&a.ctr->n_right, s_tot[3]);
-    atomicAdd(&a.ctr->n_killed, s_tot[4]);
+    atomicAdd(&ctr->events, s_tot[0]);
+    atomicAdd(&ctr->collisions, s_tot[1]);
+    atomicAdd(&ctr->n_left, s_tot[2]);
+    atomicAdd(&ctr->n_right, s_tot[3]);
+    atomicAdd(&ctr->n_killed, s_tot[4]);
   }
   if (LDS_TALLY) {
+    unsigned long long *fx = a.fx;
     for (int c = threadIdx.x; c < m; c += blockDim.x) {
-      mc_fx v;
-      v.lo = t_lo[c];
-      v.hi = t_hi[c];
-      mc_fx_atomic_add(&a.fx[c], &a.fx[m + c], v);
+      mc_fx t;
+      t.lo = t_lo[c];
+      t.hi = t_hi[c];
+      mc_fx_atomic_add(&fx[c], &fx[m + c], t);
     }
   }
 }
@@ -322,8 +605,8 @@ int launch_one(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_optio
   if (lds > 65536)
     (void)hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-  int block = o.block_size > 0 ? o.block_size : (lds > 81920 ? 1024 : 256);
-  if (block > 1024 || block % 64)
+  int block = o.block_size > 0 ? o.block_size : (lds > 81920 ? EVENT_MAX_BLOCK : 256);
+  if (block > EVENT_MAX_BLOCK || block % 64)
     return mc_fail(ctx, MCGPU_ERR_ARG, "block_size must be a multiple of 64 and <= 1024");
   int grid = o.grid_blocks;
   if (grid <= 0) {
@@ -332,8 +615,12 @@ int launch_one(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_optio
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     grid = per_cu * ctx->num_cus;
   }
-  /* particles per lane: each wave's pool holds 64 * ppl slots */
-  const int ppl = o.particles_per_lane > 0 ? o.particles_per_lane : 8;
+  /* particles per lane: each wave's pool holds 64 * ppl slots.  Four, the
+     fewest that keep four chunks streaming through the stages: the pools
+     (about 100 MB on a full chip) then stay in the Infinity Cache (measured
+     on the 10,000-cell slab at 1e8 particles: 4 slots 7.49 s, 5 7.59 s,
+     6 7.78 s, 8 8.00 s, 12 8.08 s) */
+  const int ppl = o.particles_per_lane > 0 ? o.particles_per_lane : 4;
   const int P = ppl * 64;
   int64_t per_block = (a.n + grid - 1) / grid;
   if (grid > 1 && per_block < block) { /* small banks: fewer blocks */

@@ -205,6 +205,44 @@ __device__ __forceinline__ double mc_event(mc_state &p, const mc_cell *cells, in
   return dw;
 }
 
+/* mc_event in two halves, for a kernel that loads the exp table row one
+   step before it evaluates (the event kernel's pipeline).  mc_event_decide
+   is mc_event up to the exp argument: the draw, the edge distance (IEEE
+   division), the collision decision (certified crossing, else the correctly
+   rounded log and division), the collision's new x and mu, or the crossing's
+   x; it returns -sig_a * di and the new cell index.  The caller then forms
+   dw = (1 - mc_exp_k_row(arg, row, K)) * wmc, wmc -= dw, index = index_new,
+   as mc_event does.  The operations and their order are mc_event's. */
+__device__ __forceinline__ double mc_event_decide(mc_state &p, const mc_cell cell, const double dx,
+                                                  const mc_coef &K, bool &collided,
+                                                  int &index_new) {
+  const double h = mc_rnd_real(p.seed);
+  const bool back = p.mu < 0;
+  index_new = back ? p.index - 1 : p.index + 1;
+  const double x_new_edge = (back ? p.index : p.index + 1) * dx;
+  const double q_edge = (x_new_edge - p.x) / p.mu;
+  const double di_edge = (p.mu < -MC_EPS || MC_EPS < p.mu) ? q_edge : MC_MAXREAL;
+  const double reach = cell.sig_i * di_edge;
+  const int sure_cross =
+      (int)!(cell.sig_i > MC_EPS) | (int)((1.0 - h) > mc_fma(reach, 0x1p-40, reach));
+  double di = di_edge;
+  collided = false;
+  if (!sure_cross) {
+    const double nlog = -mc_log_k(h, &K);
+    const double dq = cell.sig_i > MC_EPS ? nlog / cell.sig_i : MC_MAXREAL;
+    collided = dq < di_edge;
+    if (collided) di = dq;
+  }
+  if (collided) {
+    index_new = p.index;
+    p.x += di * p.mu;
+    p.mu = 2 * mc_rnd_real(p.seed) - 1;
+  } else {
+    p.x = x_new_edge;
+  }
+  return -cell.sig_a * di;
+}
+
 /* src/layer.cpp:195-197: the history continues while this holds */
 /* index_start <= index < index_start + m as one unsigned compare (the
    difference wraps to a large value below index_start) */
@@ -245,24 +283,32 @@ __device__ __forceinline__ unsigned long long mc_wave_sum(unsigned long long v) 
 /* SIGNED: terms of either sign are common (adjoint); otherwise negative terms
    take the general conversion (same result, fewer instructions for the
    transport's non-negative deposits). */
-template <bool SIGNED = false, bool HI_COND = false>
-__device__ __forceinline__ void mc_tally(unsigned long long *lo, unsigned long long *hi,
-                                         uint32_t *flags, unsigned int *status, int c, double dw) {
+/* on_flags(g) records the flag word g of a term the general conversion
+   flagged (never called for the common terms) */
+template <bool SIGNED = false, bool HI_COND = false, typename OnFlags>
+__device__ __forceinline__ void mc_tally_with(unsigned long long *lo, unsigned long long *hi, int c,
+                                              double dw, OnFlags on_flags) {
   mc_fx v;
   int fast;
   MC_FX_FAST(!SIGNED, dw, v, fast);
   if (!fast) { /* flags can only come from the general conversion */
     const mc_fx_flagged g = mc_fx_from_double_general(dw);
     v = g.v;
-    if (g.flags) {
-      atomicOr(&flags[c], g.flags);
-      if (g.flags & 8u) atomicOr(status, 2u);
-    }
+    if (g.flags) on_flags(g.flags);
   }
   if (SIGNED)
     mc_fx_atomic_add(&lo[c], &hi[c], v);
   else
     mc_fx_atomic_add_always<HI_COND>(&lo[c], &hi[c], v);
+}
+
+template <bool SIGNED = false, bool HI_COND = false>
+__device__ __forceinline__ void mc_tally(unsigned long long *lo, unsigned long long *hi,
+                                         uint32_t *flags, unsigned int *status, int c, double dw) {
+  mc_tally_with<SIGNED, HI_COND>(lo, hi, c, dw, [&](uint32_t g) {
+    atomicOr(&flags[c], g);
+    if (g & 8u) atomicOr(status, 2u);
+  });
 }
 
 /* Gradient terms of one replayed event for the target T = sum g_c tally_c

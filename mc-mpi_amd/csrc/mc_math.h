@@ -466,4 +466,50 @@ MC_FN double mc_exp(double x) {
   return mc_exp_k(x, &K);
 }
 
+/* mc_exp_k in two halves, for kernels that load the table row one step
+   before they evaluate: mc_exp_lookup(x) reads the row mc_exp_k reads for x
+   (any x, NaN included, gives a row of the table), and
+   mc_exp_k_row(x, row, K) == mc_exp_k(x, K) bit for bit: the same
+   operations, the row taken from the argument. */
+typedef struct mc_exp_row {
+  double th, tl;
+} mc_exp_row;
+
+MC_FN mc_exp_row mc_exp_lookup(double x) {
+  const double shifted = mc_fma(x, MC_INV_LN2_1024, 0x1.8p52);
+  const int j = (int)(uint32_t)mc_asu64(shifted) & 1023;
+  mc_exp_row R;
+  R.th = mc_exp_tab10[j][0];
+  R.tl = mc_exp_tab10[j][1];
+  return R;
+}
+
+MC_FN double mc_exp_k_row(double x, const mc_exp_row *R, const mc_coef *K) {
+  if (!(__builtin_fabs(x) < 708.3)) return mc_exp_special(x);
+  const double shifted = mc_fma(x, MC_INV_LN2_1024, 0x1.8p52);
+  const double kd = shifted - 0x1.8p52;
+  const int k = (int)(uint32_t)mc_asu64(shifted);
+  const double th = R->th, tl = R->tl;
+  /* from here as mc_exp_k (the error analysis there applies unchanged) */
+  const double t = mc_fma(-kd, MC_LN2_1024_A, x);
+  const double rh = mc_fma(-kd, MC_LN2_1024_B, t);
+  const double rl = mc_fma(-kd, MC_LN2_1024_B, t - rh);
+  double c = mc_fma(rh, K->e5, K->e4);
+  c = mc_fma(rh, c, K->e3);
+  c = mc_fma(rh, c, 0.5);
+  const double b = (rh * rh) * c;
+  double eh, el;
+  mc_fast_two_sum(1.0, rh, &eh, &el);
+  el = el + (rl + b);
+  double yh = th * eh;
+  double yl = mc_fma(th, eh, -yh);
+  yl = yl + mc_fma(th, el, tl * eh);
+  mc_fast_two_sum(yh, yl, &yh, &yl);
+  if (mc_fma(yl, K->ziv, yh) == yh) return __builtin_ldexp(yh, k >> 10);
+  if (__builtin_fabs(x) < 0x1p-30) return mc_exp_tiny(x);
+  int kk;
+  mc_exp_dd(x, 0.0, &yh, &yl, &kk);
+  return (yh + yl) * mc_pow2(kk);
+}
+
 #endif /* MC_MATH_H */

@@ -17,6 +17,13 @@ SHIM = r"""
 #include "mc_math.h"
 void pm_log(const double *x, long n, double *y) { for (long i = 0; i < n; ++i) y[i] = mc_log(x[i]); }
 void pm_exp(const double *x, long n, double *y) { for (long i = 0; i < n; ++i) y[i] = mc_exp(x[i]); }
+void pm_exp_row(const double *x, long n, double *y) {
+  const mc_coef K = mc_coef_default();
+  for (long i = 0; i < n; ++i) {
+    const mc_exp_row R = mc_exp_lookup(x[i]);
+    y[i] = mc_exp_k_row(x[i], &R, &K);
+  }
+}
 """
 
 
@@ -30,7 +37,7 @@ def pm(tmp_path_factory):
                     "-I", os.path.join(ROOT, "mc-mpi_amd", "csrc"), "-o", str(so), str(src), "-lm"],
                    check=True)
     L = C.CDLL(str(so))
-    for f in (L.pm_log, L.pm_exp):
+    for f in (L.pm_log, L.pm_exp, L.pm_exp_row):
         f.argtypes = [C.c_void_p, C.c_long, C.c_void_p]
 
     def run(which, x):
@@ -105,6 +112,22 @@ def test_exp_near_multiples_of_ln2_over_128(pm, oracle):
         + np.concatenate([off, off])
     x = x[(x > -708.3) & (x < 709.7)]
     assert np.all(_same(pm("exp", x), oracle.math_eval("exp", x)))
+
+
+def test_exp_in_two_halves_equals_exp(pm, oracle):
+    """mc_exp_lookup + mc_exp_k_row (the row loaded a step ahead, as the
+    event kernel does) give mc_exp's result bit for bit, and so the correctly
+    rounded one, on every kind of argument the tests above use."""
+    rng = np.random.default_rng(8)
+    k = np.arange(1, 20_001, dtype=np.float64)
+    x = np.concatenate([-rng.random(300_000) * 10.0 ** rng.uniform(-18, 2.9, 300_000),
+                        (rng.random(200_000) - 0.5) * 1500.0, k * 2.0 ** -53, -k * 2.0 ** -60,
+                        rng.integers(-1_040_000, 1_048_000, 100_000) * (np.log(2.0) / 1024.0),
+                        [0.0, -0.0, np.inf, -np.inf, np.nan, -745.14, 709.79, -708.3, 708.3,
+                         2.0 ** -54, 5e-324, -5e-324]])
+    got = pm("exp_row", x)
+    assert np.all(_same(got, pm("exp", x)))
+    assert np.all(_same(got, oracle.math_eval("exp", x)))
 
 
 def test_log_rows_next_to_one(pm, oracle):

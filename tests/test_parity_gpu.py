@@ -219,6 +219,24 @@ def test_launch_shape_does_not_change_results(ctx, oracle, opts):
     _assert_tally(w, o)
 
 
+@pytest.mark.parametrize("block", [1088, 2048, 96 + 1])
+def test_event_block_size_limit(ctx, block):
+    """The event kernel takes at most 1024 threads per block, in multiples of
+    64; a larger or ragged block is an argument error, with nothing run."""
+    import mcgpu
+    wl = _workload("slab100", 1_000)
+    bank = wl.bank.copy()
+    w = np.zeros(wl.m)
+    _, _, rc = ctx.simulate(bank, wl.sigs, wl.absorption_rates, w, wl.min_index, wl.max_index,
+                            wl.dx, wl.min_weight,
+                            options=mcgpu.make_options("event", block_size=block),
+                            allow=(mcgpu.ERR_ARG,))
+    assert rc == mcgpu.ERR_ARG
+    assert "1024" in ctx.error()
+    assert np.array_equal(bank, wl.bank)
+    assert not w.any()
+
+
 def _layer_case(oracle, m, n, seed=0, start=0, x_mode="face"):
     rng = np.random.default_rng(seed)
     from mcgpu import configs
