@@ -32,14 +32,18 @@
 
 namespace {
 
-#define DEAD_ID 0xFFFFFFFFu
-/* Bit 31 of a live slot's id: the particle's last event crossed a cell edge,
-   so its x is the edge its cell index and direction imply,
-   (mu < 0 ? index + 1 : index) * dx -- the very product the crossing stored
-   (src/layer.cpp:141-143,168-171) -- and the slot's x field is neither
-   written nor read.  Most events are crossings, so a pass moves 36 bytes per
-   particle each way instead of 44.  Particle ids stay below 2^31 - 1. */
+/* A slot's cell word: the particle's cell as an offset into the layer
+   (index - index_start, in [0, m)) in bits 0..30, and in bit 31 whether its
+   last event crossed a cell edge.  On an edge, x is the edge its cell and
+   direction imply, (mu < 0 ? index + 1 : index) * dx -- the very product the
+   crossing stored (src/layer.cpp:141-143,168-171) -- and the slot's x field
+   is neither written nor read.  All ones marks a dead slot.  A pass reads
+   the cell word, seed, mu, wmc (and x off an edge) and writes back what the
+   event changed; the particle id and the pass its event count started from
+   are read only by compaction passes, when the event cap can bind, or when
+   the history ends. */
 #define ON_EDGE 0x80000000u
+#define DEAD_CW 0xFFFFFFFFu
 
 /* A block's pool: two halves of P slots each, structure of arrays.  Field
    k of slot j in half h lives at base + h*HALF + OFF_k*P + j*size_k, with
@@ -54,7 +58,7 @@ struct pool_ref {
   __device__ __forceinline__ double *x() const { return reinterpret_cast<double *>(b + 8 * (size_t)P); }
   __device__ __forceinline__ double *mu() const { return reinterpret_cast<double *>(b + 16 * (size_t)P); }
   __device__ __forceinline__ double *wmc() const { return reinterpret_cast<double *>(b + 24 * (size_t)P); }
-  __device__ __forceinline__ int32_t *index() const { return reinterpret_cast<int32_t *>(b + 32 * (size_t)P); }
+  __device__ __forceinline__ uint32_t *cw() const { return reinterpret_cast<uint32_t *>(b + 32 * (size_t)P); }
   __device__ __forceinline__ uint32_t *id() const { return reinterpret_cast<uint32_t *>(b + 36 * (size_t)P); }
   /* the pass at which the particle's event count was 0: every pass gives
      every live slot one event, so the count before pass k is k - t0 */
@@ -87,17 +91,18 @@ template <typename T> __device__ __forceinline__ void st_nt(T *p, T v) {
 #endif
 }
 
+/* a new particle (cell word cw, not on an edge) into slot j */
 template <bool COLL>
 __device__ __forceinline__ void put(const pool_ref &h, int j, const mc_state &p, uint32_t id,
-                                    uint32_t t0, uint32_t col, bool on_edge) {
+                                    uint32_t t0, uint32_t cw) {
   st_nt(h.seed() + j, p.seed);
-  if (!on_edge) st_nt(h.x() + j, p.x);
+  st_nt(h.x() + j, p.x);
   st_nt(h.mu() + j, p.mu);
   st_nt(h.wmc() + j, p.wmc);
-  st_nt(h.index() + j, p.index);
-  st_nt(h.id() + j, on_edge ? (id | ON_EDGE) : id);
+  st_nt(h.cw() + j, cw);
+  st_nt(h.id() + j, id);
   st_nt(h.t0() + j, t0);
-  if (COLL) st_nt(h.col() + j, col);
+  if (COLL) st_nt(h.col() + j, 0u);
 }
 
 /* ------------------------------------------------------------ the pipeline
@@ -312,7 +317,7 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
       own_sreg(a.max_events > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)a.max_events);
   const double dx = L.dx;
   /* field offsets in a half (bytes) */
-  const int F_X = 8 * P, F_MU = 16 * P, F_WMC = 24 * P, F_INDEX = 32 * P, F_ID = 36 * P,
+  const int F_X = 8 * P, F_MU = 16 * P, F_WMC = 24 * P, F_CW = 32 * P, F_ID = 36 * P,
             F_T0 = 40 * P, F_COL = 44 * P;
   const unsigned ce = (unsigned)compact_every;
 
@@ -320,6 +325,16 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
   int n_pool = 0; /* slots in use in the current half (wave-uniform) */
   bool queue_done = false;
   bool capped = false;
+  /* a finished history (ev events, col collisions) to its bank slot */
+  auto finish = [&](const mc_state &p, uint32_t id, uint32_t ev, uint32_t col) {
+    int code = mc_exit_code(p, L);
+    if (mc_alive(p, L)) { /* alive: stopped by the event cap */
+      capped = true;
+      code = 2;
+    }
+    exit_state(fresh(v), (int64_t)id, p, code, ev, col);
+    atomicAdd(&s_tot[code == -1 ? 2 : (code == 1 ? 3 : 4)], code == 2 ? 0ull : 1ull);
+  };
 
   for (unsigned pass = 0;;) {
     /* ---- refill from the work queue (wave-uniform condition) */
@@ -343,8 +358,14 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
       const pool_ref h{mine + cur * half_bytes, P};
       for (int k = lane; k < take; k += 64) {
         mc_state p;
-        refill_state(fresh(v), (int64_t)(base + k), p);
-        put<COLL>(h, n_pool + k, p, (uint32_t)(base + k), (uint32_t)pass, 0u, false);
+        const uint32_t id = (uint32_t)(base + k);
+        refill_state(fresh(v), (int64_t)id, p);
+        if (mc_alive(p, L) && max_ev > 0) {
+          put<COLL>(h, n_pool + k, p, id, (uint32_t)pass, (uint32_t)(p.index - L.index_start));
+        } else { /* no event to apply: finished as it came, its slot dead */
+          finish(p, id, 0u, 0u);
+          h.cw()[n_pool + k] = DEAD_CW;
+        }
       }
       n_pool += take;
       /* slots written by one lane are read by another lane of this wave */
@@ -362,6 +383,9 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
     const unsigned last = nch >= 4 ? pass + (ce - pass % ce) % ce : pass;
     const int total = (int)(last - pass + 1) * nch;
     const bool ends_compacting = last % ce == 0;
+    /* the event cap can bind in this segment (a count after pass q is at
+       most q + 1): the passes then read every slot's count origin */
+    const bool cap_live = (unsigned long long)last + 1 >= max_ev;
     const rsrc_t src = half_rsrc(mine + cur * half_bytes, P);
     const rsrc_t dst = half_rsrc(mine + (cur ^ 1) * half_bytes, P);
     int wpos = 0; /* wave-uniform output position of the compaction */
@@ -369,9 +393,9 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
     unsigned a_pass = pass; /* and pass */
     unsigned int lane_ev = 0, lane_col = 0;
 
-    /* stage registers: A -> B */
-    uint32_t a_id = 0, a_t0 = 0;
-    int32_t a_index = 0;
+    /* stage registers (id and t0 as loaded: zero unless the chunk's pass
+       compacts or the cap can bind) -- A -> B */
+    uint32_t a_id = 0, a_t0 = 0, a_cw = 0;
     bool a_valid = false;
     int a_slot = 0;
     unsigned a_q = 0;
@@ -380,7 +404,7 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
     mc_cell b_cell;
     b_cell.sig_a = 0.0;
     b_cell.sig_i = 0.0;
-    uint32_t b_id = 0, b_ev = 0, b_col = 0;
+    uint32_t b_id = 0, b_t0 = 0, b_col = 0;
     int32_t b_index = 0;
     bool b_live = false, b_on_edge = false;
     int b_slot = 0;
@@ -392,7 +416,7 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
     c_p.mu = 0.0;
     c_p.wmc = 0.0;
     c_p.index = 0;
-    uint32_t c_id = 0, c_ev = 0, c_col = 0;
+    uint32_t c_id = 0, c_t0 = 0, c_col = 0;
     int32_t c_index_new = 0;
     int c_cell = 0;
     bool c_live = false, c_event = false, c_coll = false, c_was_on_edge = false;
@@ -409,6 +433,8 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
         /* the pass of this chunk compacts: the segment's last pass, when
            that is a compaction pass (passes before it are not) */
         const bool compact = ends_compacting && c_q == last;
+        /* id and t0 were loaded for this chunk */
+        const bool aux = cap_live || compact;
         bool keep = false, on_edge = false;
         if (c_live) {
           if (c_event) {
@@ -417,7 +443,6 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
             c_p.wmc -= dw;
             c_p.index = c_index_new;
             on_edge = !c_coll;
-            ++c_ev;
             ++lane_ev;
             c_col += c_coll ? 1u : 0u;
             lane_col += c_coll ? 1u : 0u;
@@ -430,24 +455,25 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
                                       __HIP_MEMORY_SCOPE_AGENT);
             });
           }
-          keep = mc_alive(c_p, L) && c_ev < max_ev;
+          const uint32_t done = c_event ? 1u : 0u;
+          keep = mc_alive(c_p, L) && (!cap_live || c_q - c_t0 + done < max_ev);
           if (!keep) { /* history finished: write it to its bank slot */
-            int code = mc_exit_code(c_p, L);
-            if (mc_alive(c_p, L)) {
-              capped = true;
-              code = 2;
+            uint32_t id = c_id, t0 = c_t0;
+            if (!aux) { /* not loaded for this pass: read them now */
+              id = ld32(src, 4 * c_slot, F_ID);
+              t0 = ld32(src, 4 * c_slot, F_T0);
             }
-            exit_state(fresh(v), (int64_t)c_id, c_p, code, c_ev, c_col);
-            atomicAdd(&s_tot[code == -1 ? 2 : (code == 1 ? 3 : 4)], code == 2 ? 0ull : 1ull);
+            finish(c_p, id, c_q - t0 + done, c_col);
           }
         }
         /* the write-back: the offsets per path, then the same eight stores
            on every path (a store on one path only would make the compiler
            assume fewer operations in flight after the branches and wait
            longer at the next stage); masked lanes aimed beyond the pool */
-        const uint32_t id_val = keep ? (on_edge ? (c_id | ON_EDGE) : c_id) : DEAD_ID;
+        const uint32_t cw_val =
+            keep ? ((uint32_t)(c_p.index - L.index_start) | (on_edge ? ON_EDGE : 0u)) : DEAD_CW;
         const bool w_x = keep && !on_edge; /* x is stored unless implicit */
-        int o_seed, o_x, o_mu, o_index, o_id, o_t0, o_col;
+        int o_seed, o_x, o_mu, o_cw, o_aux, o_col;
         if (compact) { /* wave-uniform: survivors packed into the other half */
           const unsigned long long km = __ballot(keep);
           const int pos = wpos + mc_mbcnt64(km);
@@ -455,13 +481,13 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
           o_seed = keep ? 8 * pos : OOB;
           o_x = w_x ? 8 * pos : OOB;
           o_mu = o_seed;
-          o_index = o_id = o_t0 = o_col = keep ? 4 * pos : OOB;
+          o_cw = o_aux = o_col = keep ? 4 * pos : OOB;
         } else { /* in place: only what the event changed */
           o_seed = keep ? 8 * c_slot : OOB;
           o_x = o_mu = w_x ? 8 * c_slot : OOB; /* a collision's x and mu */
-          o_index = (keep && on_edge) ? 4 * c_slot : OOB;
-          o_id = (c_live && (!keep || on_edge != c_was_on_edge)) ? 4 * c_slot : OOB;
-          o_t0 = OOB; /* t0 stays */
+          /* a crossing's new cell, a collision's cleared edge bit, a death */
+          o_cw = (c_live && (!keep || on_edge || c_was_on_edge)) ? 4 * c_slot : OOB;
+          o_aux = OOB; /* id and t0 stay */
           o_col = keep ? 4 * c_slot : OOB;
         }
         const rsrc_t r = compact ? dst : src;
@@ -469,9 +495,9 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
         st64(r, o_x, F_X, as_u64(c_p.x));
         st64(r, o_mu, F_MU, as_u64(c_p.mu));
         st64(r, o_seed, F_WMC, as_u64(c_p.wmc));
-        st32(r, o_index, F_INDEX, (uint32_t)c_p.index);
-        st32(r, o_id, F_ID, id_val);
-        st32(r, o_t0, F_T0, c_q + 1u - c_ev);
+        st32(r, o_cw, F_CW, cw_val);
+        st32(r, o_aux, F_ID, c_id);
+        st32(r, o_aux, F_T0, c_t0);
         if (COLL) st32(r, o_col, F_COL, c_col);
       }
       /* the stages in this order: each reads its inputs before the next one
@@ -495,14 +521,14 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
         int index_new = b_index;
         double arg = 0.0;
         const int c = b_index - L.index_start;
-        if (b_live && mc_alive(p, L) && b_ev < max_ev) {
+        if (b_live && mc_alive(p, L) && (!cap_live || b_q - b_t0 < max_ev)) {
           event = true;
           arg = mc_event_decide(p, cell, dx, K, coll, index_new);
         }
         c_row = mc_exp_lookup(arg); /* every lane: a row of the table */
         c_p = p;
         c_id = b_id;
-        c_ev = b_ev;
+        c_t0 = b_t0;
         c_col = col0;
         c_index_new = index_new;
         c_cell = c;
@@ -517,8 +543,8 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
       __builtin_amdgcn_sched_barrier(0);
       /* ======== B: chunk t-1 */
       {
-        const bool live = a_valid && a_id != DEAD_ID;
-        const bool on_edge = live && (a_id & ON_EDGE) != 0;
+        const bool live = a_valid && a_cw != DEAD_CW;
+        const bool on_edge = live && (a_cw & ON_EDGE) != 0;
         const int o8 = live ? 8 * a_slot : OOB;
         b_seed = ld64(src, o8, 0);
         b_mu = ld64(src, o8, F_MU);
@@ -526,16 +552,15 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
         b_x = ld64(src, (live && !on_edge) ? 8 * a_slot : OOB, F_X);
         if (COLL) b_col = ld32(src, live ? 4 * a_slot : OOB, F_COL);
         /* the cell, at an index inside the layer on every lane */
-        const unsigned cu = (unsigned)(a_index - L.index_start);
+        const unsigned cu = a_cw & ~ON_EDGE;
         const unsigned ci = cu < (unsigned)m ? cu : 0u;
         if (LDS_CELLS)
           b_cell = cells_l[ci];
         else
           b_cell = ld_cell(cells_g + ci);
-        b_id = a_id & ~ON_EDGE;
-        b_index = (int32_t)opq32((uint32_t)a_index);
-        b_ev = a_q - a_t0;
-        asm volatile("" : "+v"(b_id), "+v"(b_ev));
+        b_id = opq32(a_id);
+        b_t0 = opq32(a_t0);
+        b_index = (int32_t)cu + L.index_start;
         b_live = live;
         b_on_edge = on_edge;
         b_slot = a_slot;
@@ -547,10 +572,13 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
         const bool active = t < total; /* wave-uniform */
         const int slot = a_j * 64 + lane;
         a_valid = active && slot < n_pool;
+        /* id and t0 only for a compaction pass or under a binding cap */
+        const bool aux = cap_live || (ends_compacting && a_pass == last);
         const int o4 = a_valid ? 4 * slot : OOB;
-        a_id = ld32(src, o4, F_ID);
-        a_index = (int32_t)ld32(src, o4, F_INDEX);
-        a_t0 = ld32(src, o4, F_T0);
+        const int o4x = aux ? o4 : OOB;
+        a_cw = ld32(src, o4, F_CW);
+        a_id = ld32(src, o4x, F_ID);
+        a_t0 = ld32(src, o4x, F_T0);
         a_slot = slot;
         a_q = a_pass;
         if (++a_j == nch) { /* chunk index and pass of the next chunk */
