@@ -7,22 +7,23 @@
  * bank, and the bank is compacted between passes, so all lanes of a wave
  * always work on the same step of different particles.
  *
- * Layout and schedule (one persistent block per CU-slot, no launch per pass):
- *  - each block owns a particle pool in HBM: two halves (ping-pong) of P
- *    slots in structure-of-arrays form (seed, x, mu, wmc, index, id, the
- *    pass its event count started from, collision counter), so every field
- *    is a coalesced 512-byte wave access;
- *  - a pass streams the live part of the current half: load, one event,
- *    tally, then the survivors are compacted into the other half -- a wave
- *    ballot + mbcnt prefix gives each survivor its rank, one LDS atomic per
- *    wave reserves the wave's range; finished histories are written to their
- *    original bank slot (the particle id), so compaction order is free;
- *  - with compaction interval c > 1 (default 16), c-1 passes out of c write
+ * Layout and schedule (persistent blocks, no launch per pass):
+ *  - each wave owns a particle pool in device memory: two halves (ping-pong)
+ *    of P = 64 * particles_per_lane slots in structure-of-arrays form (seed,
+ *    x, mu, wmc, cell word, id, the pass its event count started from,
+ *    collision counter), so every field is a coalesced wave access;
+ *  - a pass gives every live slot one event, 64 slots (a chunk) at a time,
+ *    through a four-stage software pipeline (below); the survivors of a
+ *    compaction pass are packed into the other half -- a wave ballot +
+ *    mbcnt prefix gives each survivor its rank; finished histories are
+ *    written to their original bank slot (the particle id), so compaction
+ *    order is free;
+ *  - with compaction interval c > 1 (default 64), c-1 passes out of c write
  *    survivors back in place -- only the fields their event changed -- and
  *    mark finished slots dead instead of compacting;
- *  - when the pool has room for a block-width of particles, it is refilled
- *    from the call's bank through the global work queue (one atomic per
- *    refill), so blocks stay full until the bank is exhausted;
+ *  - when the pool has room for a chunk, it is refilled from the call's bank
+ *    through the global work queue (one atomic per refill), so pools stay
+ *    full until the bank is exhausted;
  *  - the exact fixed-point tally lives in LDS for the block's lifetime and is
  *    added to the global tally once (same arithmetic as the history kernel).
  * Results are bit-identical to the history kernel and the oracle: histories
@@ -266,13 +267,19 @@ __device__ __forceinline__ void exit_state(const sargs &v, int64_t i, const mc_s
    vector registers and every pool access in a loop over its lanes. */
 constexpr int EVENT_MAX_BLOCK = 1024;
 
+/* the kernel's own LDS (block totals, the arguments), an upper bound: the
+   tally and cells take what is left of the block's */
+constexpr size_t EVENT_STATIC_LDS = 2048;
+static_assert(5 * 8 + sizeof(mc_transport_args) + 64 <= EVENT_STATIC_LDS, "static LDS bound");
+
 template <bool LDS_TALLY, bool LDS_CELLS, bool COLL>
 __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
                                                            unsigned char *pools, int P,
                                                            int compact_every) {
   extern __shared__ __align__(16) unsigned char smem[];
-  /* coefficients as constants: scalar moves at their uses, no registers */
-  const mc_coef K = mc_coef_default();
+  /* the log and exp coefficients in vector registers (as constants each use
+     would take two scalar moves) */
+  const mc_coef K = mc_coef_live();
   /* block totals: events, collisions, left, right, killed */
   __shared__ unsigned long long s_tot[5];
   __shared__ mc_transport_args s_a;
@@ -655,10 +662,12 @@ int launch_one(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_optio
     grid = (int)((a.n + block - 1) / block);
     if (grid < 1) grid = 1;
   }
-  /* compaction every 16th pass: the passes between write back only what
-     changed (measured on the 10,000-cell slab: every pass 10.13 s, every 4th 9.25 s,
-     8th 9.11 s, 16th 9.04 s, 32nd 9.02 s) */
-  const int c = o.compaction_interval > 0 ? o.compaction_interval : 16;
+  /* compaction every 64th pass: the passes between write back only what
+     changed, and a segment's chunks stream through the stages without a
+     break (measured on the 10,000-cell slab at 1e8 particles, four slots per
+     lane: every 8th pass 7.45 s, 16th 7.13 s, 32nd 6.98 s, 64th 6.91 s,
+     128th 6.93 s, 256th 6.96 s, 1024th 7.10 s) */
+  const int c = o.compaction_interval > 0 ? o.compaction_interval : 64;
   const size_t waves = (size_t)grid * (size_t)(block / 64);
   int rc = mc_ensure(ctx, &ctx->work, &ctx->work_bytes, waves * 2 * SLOT_BYTES * (size_t)P);
   if (rc) return rc;
@@ -675,7 +684,7 @@ int mc_launch_event(mcgpu_context *ctx, const mc_transport_args &a, const mcgpu_
     return mc_fail(ctx, MCGPU_ERR_ARG, "event kernel: bank larger than 2^31-2 particles");
   const size_t m = (size_t)a.L.m;
   const size_t tally = 16 * m, cells = sizeof(mc_cell) * m;
-  const bool lds_tally = tally <= ctx->lds_per_block;
+  const bool lds_tally = tally + EVENT_STATIC_LDS <= ctx->lds_per_block;
   const bool lds_cells = lds_tally && tally + cells <= 65536;
   /* collision counters ride in the pool only when the caller asked for them
      per particle; the call's total is kept either way */

@@ -253,10 +253,12 @@ def _layer_case(oracle, m, n, seed=0, start=0, x_mode="face"):
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
-@pytest.mark.parametrize("m", [12_000, 5_000, 64])
+@pytest.mark.parametrize("m", [12_000, 10_230, 10_100, 5_000, 64])
 def test_tally_paths_lds_and_global(ctx, oracle, m, kernel):
     """m = 12000 exceeds one block's LDS (global 128-bit tally), 5000 keeps
-    the tally in LDS with the cross sections in L2, 64 keeps both in LDS."""
+    the tally in LDS with the cross sections in L2, 64 keeps both in LDS.
+    10,100 and 10,230 sit either side of the event kernel's limit, a 160 KiB
+    block less its own 2 KiB (163,648 and 163,680 bytes of tally + 2,048)."""
     _say(f"tally path m={m}")
     sigs, ar, dx, bank = _layer_case(oracle, m, 300 if m > 1000 else 3000, seed=m)
     o = oracle.simulate_bank(64, 0, m, dx, 1e-3, sigs, ar, bank, nthread=8)
