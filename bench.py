@@ -52,8 +52,8 @@ BYTES_PER_EVENT = 72  # algorithmic: the 36-byte particle state read + written p
 #    an upper bound on HBM bytes, since the Infinity Cache (MALL) in front of
 #    HBM can serve them;
 #  - VALU wave-instructions (SQ_INSTS_VALU).
-MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0616, "event": 55.7}
-MEASURED_VALU_PER_EVENT = {"history": 1.668, "event": 3.205}
+MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0616, "event": 20.98}
+MEASURED_VALU_PER_EVENT = {"history": 1.668, "event": 3.542}
 # gfx950 issues one wave64 VALU instruction per SIMD every 4 cycles:
 # 256 CUs x 4 SIMDs x 2.4 GHz / 4 = 6.144e11 wave-instructions/s.
 VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
@@ -71,15 +71,16 @@ def valu_rate(kernel, events, seconds):
 def roofline(kernel, events, seconds):
     """The roofline block of one plain-kernel launch (events over seconds).
 
+    Both kernels are bound by VALU issue: `frac` is the issue rate (measured
+    wave-instructions per event x events/s) over gfx950's 6.144e11
+    wave-instructions/s, and the bytes are a side field (`hbm`).
     history kernel: the state stays in registers (about 0.06 B/event of
-      traffic), so HBM does not bound it; it is bound by VALU issue.  `frac`
-      is the issue rate (measured wave-instructions per event x events/s)
-      over gfx950's 6.144e11 wave-instructions/s; the bytes are a side field.
-    event kernel: bound by memory.  `frac` is the measured bytes per event
-      (L2-miss bytes, FETCH_SIZE + WRITE_SIZE) x events/s over 8 TB/s.  Its
-      ping-pong pools (about 180 MB) are sized to stay in the 256 MB MALL,
-      so those bytes are not all HBM bytes; the 72 algorithmic B/event
-      yardstick of the event formulation is reported beside it."""
+      traffic).
+    event kernel: every event reloads and stores its particle, but the
+      fields a pass touches (about 3.7 MB per XCD for the four-slot pools)
+      stay in L2: about 4 B/event read and 17 B/event written reach the
+      memory fabric, against the 72 B/event algorithmic yardstick of the
+      event formulation (the 36-byte state read and written)."""
     measured_b = MEASURED_TRAFFIC_PER_EVENT.get(kernel)
     traffic = None if measured_b is None else measured_b * events
     alg_gbs = events * BYTES_PER_EVENT / seconds / 1e9
@@ -90,23 +91,16 @@ def roofline(kernel, events, seconds):
            "measured_frac": None if traffic is None else traffic / seconds / 1e9 / HBM_PEAK_GBS,
            "peak": HBM_PEAK_GBS, "unit": "GB/s"}
     valu = valu_rate(kernel, events, seconds)
-    if kernel == "history":
-        return {"bound": "valu", "achieved": valu["achieved"], "peak": valu["peak"],
-                "unit": valu["unit"], "frac": valu["frac"], "traffic": traffic,
-                "wave_instructions_per_event": valu["wave_instructions_per_event"],
-                "hbm": hbm,
-                "note": "history kernel: one history per lane, state in registers; "
-                        "bound by FP64 VALU issue (frac = measured wave-instructions/event x "
-                        "events/s / 6.144e11); traffic = measured L2-miss bytes per launch"}
-    return {"bound": "hbm", "achieved": hbm["measured_GBs"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": hbm["measured_frac"], "traffic": traffic,
-            "algorithmic": {"bytes_per_event": BYTES_PER_EVENT, "achieved": alg_gbs,
-                            "frac": alg_gbs / HBM_PEAK_GBS},
-            "valu": valu,
-            "note": "event kernel: achieved = measured L2-miss bytes (FETCH_SIZE + WRITE_SIZE, "
-                    "copy-calibrated) per event x events/s; the per-block pools (about "
-                    "180 MB) are sized to stay in the 256 MB MALL, so these bytes are an "
-                    "upper bound on HBM bytes"}
+    note = ("history kernel: one history per lane, state in registers; "
+            if kernel == "history" else
+            "event kernel: one event per live particle per pass, state reloaded each event "
+            "from per-wave pools whose touched fields stay in L2; ")
+    return {"bound": "valu", "achieved": valu["achieved"], "peak": valu["peak"],
+            "unit": valu["unit"], "frac": valu["frac"], "traffic": traffic,
+            "wave_instructions_per_event": valu["wave_instructions_per_event"],
+            "hbm": hbm,
+            "note": note + "bound by FP64 VALU issue (frac = measured wave-instructions/event x "
+                           "events/s / 6.144e11); traffic = measured L2-miss bytes per launch"}
 
 
 def host_cores():
