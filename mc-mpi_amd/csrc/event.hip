@@ -120,9 +120,9 @@ __device__ __forceinline__ void put(const pool_ref &h, int j, const mc_state &p,
         slot (in place, or compacted into the other half);
      C  the state and the cell have arrived: the draw, the edge distance,
         the collision decision, the new x and mu; load the exp table row;
-     B  id, index and the event-count origin have arrived: load the rest of
-        the state (x only where it is not implicit) and the cell;
-     A  load id, index and the event-count origin of the next chunk.
+     B  the cell word has arrived: load the rest of the state (x only where
+        it is not implicit) and the cell;
+     A  load the cell word of the next chunk.
    Every wait of an iteration is then for operations issued in the
    iteration before.  Pool accesses are raw buffer loads and stores whose
    masked lanes take an offset beyond the pool (no memory access; loads give
@@ -400,9 +400,8 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
     unsigned a_pass = pass; /* and pass */
     unsigned int lane_ev = 0, lane_col = 0;
 
-    /* stage registers (id and t0 as loaded: zero unless the chunk's pass
-       compacts or the cap can bind) -- A -> B */
-    uint32_t a_id = 0, a_t0 = 0, a_cw = 0;
+    /* stage registers -- A -> B */
+    uint32_t a_cw = 0;
     bool a_valid = false;
     int a_slot = 0;
     unsigned a_q = 0;
@@ -411,7 +410,7 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
     mc_cell b_cell;
     b_cell.sig_a = 0.0;
     b_cell.sig_i = 0.0;
-    uint32_t b_id = 0, b_t0 = 0, b_col = 0;
+    uint32_t b_col = 0;
     int32_t b_index = 0;
     bool b_live = false, b_on_edge = false;
     int b_slot = 0;
@@ -423,7 +422,7 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
     c_p.mu = 0.0;
     c_p.wmc = 0.0;
     c_p.index = 0;
-    uint32_t c_id = 0, c_t0 = 0, c_col = 0;
+    uint32_t c_col = 0;
     int32_t c_index_new = 0;
     int c_cell = 0;
     bool c_live = false, c_event = false, c_coll = false, c_was_on_edge = false;
@@ -440,8 +439,6 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
         /* the pass of this chunk compacts: the segment's last pass, when
            that is a compaction pass (passes before it are not) */
         const bool compact = ends_compacting && c_q == last;
-        /* id and t0 were loaded for this chunk */
-        const bool aux = cap_live || compact;
         bool keep = false, on_edge = false;
         if (c_live) {
           if (c_event) {
@@ -463,24 +460,27 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
             });
           }
           const uint32_t done = c_event ? 1u : 0u;
-          keep = mc_alive(c_p, L) && (!cap_live || c_q - c_t0 + done < max_ev);
+          bool at_cap = false;
+          if (cap_live) /* wave-uniform: the count can reach the cap */
+            at_cap = c_q - ld32(src, 4 * c_slot, F_T0) + done >= max_ev;
+          keep = mc_alive(c_p, L) && !at_cap;
           if (!keep) { /* history finished: write it to its bank slot */
-            uint32_t id = c_id, t0 = c_t0;
-            if (!aux) { /* not loaded for this pass: read them now */
-              id = ld32(src, 4 * c_slot, F_ID);
-              t0 = ld32(src, 4 * c_slot, F_T0);
-            }
+            const uint32_t id = ld32(src, 4 * c_slot, F_ID);
+            const uint32_t t0 = ld32(src, 4 * c_slot, F_T0);
+            /* both loads complete on this path (t0 is used only when the
+               counts are asked for): no load still in flight leaves it */
+            asm volatile("" ::"v"(id), "v"(t0));
             finish(c_p, id, c_q - t0 + done, c_col);
           }
         }
-        /* the write-back: the offsets per path, then the same eight stores
-           on every path (a store on one path only would make the compiler
-           assume fewer operations in flight after the branches and wait
-           longer at the next stage); masked lanes aimed beyond the pool */
+        /* the write-back: the offsets per path, then the same stores on every
+           path (a store on one path only would make the compiler assume
+           fewer operations in flight after the branches and wait longer at
+           the next stage); masked lanes aimed beyond the pool */
         const uint32_t cw_val =
             keep ? ((uint32_t)(c_p.index - L.index_start) | (on_edge ? ON_EDGE : 0u)) : DEAD_CW;
         const bool w_x = keep && !on_edge; /* x is stored unless implicit */
-        int o_seed, o_x, o_mu, o_cw, o_aux, o_col;
+        int o_seed, o_x, o_mu, o_cw, o_col;
         if (compact) { /* wave-uniform: survivors packed into the other half */
           const unsigned long long km = __ballot(keep);
           const int pos = wpos + mc_mbcnt64(km);
@@ -488,13 +488,12 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
           o_seed = keep ? 8 * pos : OOB;
           o_x = w_x ? 8 * pos : OOB;
           o_mu = o_seed;
-          o_cw = o_aux = o_col = keep ? 4 * pos : OOB;
+          o_cw = o_col = keep ? 4 * pos : OOB;
         } else { /* in place: only what the event changed */
           o_seed = keep ? 8 * c_slot : OOB;
           o_x = o_mu = w_x ? 8 * c_slot : OOB; /* a collision's x and mu */
           /* a crossing's new cell, a collision's cleared edge bit, a death */
           o_cw = (c_live && (!keep || on_edge || c_was_on_edge)) ? 4 * c_slot : OOB;
-          o_aux = OOB; /* id and t0 stay */
           o_col = keep ? 4 * c_slot : OOB;
         }
         const rsrc_t r = compact ? dst : src;
@@ -503,9 +502,14 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
         st64(r, o_mu, F_MU, as_u64(c_p.mu));
         st64(r, o_seed, F_WMC, as_u64(c_p.wmc));
         st32(r, o_cw, F_CW, cw_val);
-        st32(r, o_aux, F_ID, c_id);
-        st32(r, o_aux, F_T0, c_t0);
         if (COLL) st32(r, o_col, F_COL, c_col);
+        if (compact) { /* the id and the count origin move with the slot (the
+                          wait for them is one pass in 64) */
+          const int o = keep ? 4 * c_slot : OOB;
+          const uint32_t id = ld32(src, o, F_ID), t0 = ld32(src, o, F_T0);
+          st32(dst, o_cw, F_ID, id);
+          st32(dst, o_cw, F_T0, t0);
+        }
       }
       /* the stages in this order: each reads its inputs before the next one
          overwrites them, so a load's destination is never copied while the
@@ -528,14 +532,18 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
         int index_new = b_index;
         double arg = 0.0;
         const int c = b_index - L.index_start;
-        if (b_live && mc_alive(p, L) && (!cap_live || b_q - b_t0 < max_ev)) {
+        bool below_cap = true;
+        if (cap_live) { /* wave-uniform: the count origin, read only here */
+          const uint32_t t0 = ld32(src, b_live ? 4 * b_slot : OOB, F_T0);
+          below_cap = b_q - t0 < max_ev;
+          asm volatile("" ::"v"(t0)); /* waited for on this path only */
+        }
+        if (b_live && mc_alive(p, L) && below_cap) {
           event = true;
           arg = mc_event_decide(p, cell, dx, K, coll, index_new);
         }
         c_row = mc_exp_lookup(arg); /* every lane: a row of the table */
         c_p = p;
-        c_id = b_id;
-        c_t0 = b_t0;
         c_col = col0;
         c_index_new = index_new;
         c_cell = c;
@@ -565,8 +573,6 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
           b_cell = cells_l[ci];
         else
           b_cell = ld_cell(cells_g + ci);
-        b_id = opq32(a_id);
-        b_t0 = opq32(a_t0);
         b_index = (int32_t)cu + L.index_start;
         b_live = live;
         b_on_edge = on_edge;
@@ -579,13 +585,7 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
         const bool active = t < total; /* wave-uniform */
         const int slot = a_j * 64 + lane;
         a_valid = active && slot < n_pool;
-        /* id and t0 only for a compaction pass or under a binding cap */
-        const bool aux = cap_live || (ends_compacting && a_pass == last);
-        const int o4 = a_valid ? 4 * slot : OOB;
-        const int o4x = aux ? o4 : OOB;
-        a_cw = ld32(src, o4, F_CW);
-        a_id = ld32(src, o4x, F_ID);
-        a_t0 = ld32(src, o4x, F_T0);
+        a_cw = ld32(src, a_valid ? 4 * slot : OOB, F_CW);
         a_slot = slot;
         a_q = a_pass;
         if (++a_j == nch) { /* chunk index and pass of the next chunk */
