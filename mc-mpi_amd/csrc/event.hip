@@ -46,10 +46,10 @@ namespace {
 #define ON_EDGE 0x80000000u
 #define DEAD_CW 0xFFFFFFFFu
 
-/* A block's pool: two halves of P slots each, structure of arrays.  Field
+/* A wave's pool: two halves of P slots each, structure of arrays.  Field
    k of slot j in half h lives at base + h*HALF + OFF_k*P + j*size_k, with
-   the 8-byte fields first.  Addresses are formed from one base pointer so
-   the compiler keeps a single scalar base per half live. */
+   the 8-byte fields first.  pool_ref addresses the fields for the refill;
+   the passes reach them through buffer resources (below). */
 constexpr size_t SLOT_BYTES = 8 * 4 + 4 * 4; /* 48 bytes per slot */
 
 struct pool_ref {
@@ -127,12 +127,15 @@ __device__ __forceinline__ void put(const pool_ref &h, int j, const mc_state &p,
    iteration before.  Pool accesses are raw buffer loads and stores whose
    masked lanes take an offset beyond the pool (no memory access; loads give
    0), and cell and table loads take a valid address on every lane, so each
-   stage issues the same memory instructions on every path: the compiler's
-   wait counts stay exact.  Passes between compactions update the slots in
-   place, so their chunks stream through the pipeline without a break (a
-   slot is written in stage D of one pass before stage A of the next pass
-   reads it, given at least four chunks); the pipeline drains at each
-   compaction and before the pool is refilled. */
+   stage issues the same memory instructions on every common path: the
+   compiler's wait counts stay exact.  The rare reads (a history's exit, a
+   compaction pass's id and count origin, a binding cap) sit in branches of
+   their own and are waited for inside them.  The passes up to the next
+   compaction update the slots in place, so their chunks stream through the
+   pipeline without a break (a slot is written in stage D of one pass before
+   stage A of the next pass reads it, given at least four chunks); the
+   pipeline drains after each compaction pass and before the pool is
+   refilled. */
 
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
