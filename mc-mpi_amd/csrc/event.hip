@@ -67,43 +67,21 @@ struct pool_ref {
   __device__ __forceinline__ uint32_t *col() const { return reinterpret_cast<uint32_t *>(b + 44 * (size_t)P); }
 };
 
-/* Pool accesses use ordinary loads and stores by default: measured on
-   MI355X (10,000-cell slab, 2e7 particles) non-temporal hints made the kernel 40 %
-   slower, as the pools (about 180 MB at 8 slots per lane) then no longer stay
-   in the Infinity Cache.  The hints remain selectable for experiments. */
-#ifndef MC_POOL_NT_LOAD
-#define MC_POOL_NT_LOAD 0
-#endif
-#ifndef MC_POOL_NT_STORE
-#define MC_POOL_NT_STORE 0
-#endif
-template <typename T> __device__ __forceinline__ T ld_nt(const T *p) {
-#if MC_POOL_NT_LOAD
-  return __builtin_nontemporal_load(p);
-#else
-  return *p;
-#endif
-}
-template <typename T> __device__ __forceinline__ void st_nt(T *p, T v) {
-#if MC_POOL_NT_STORE
-  __builtin_nontemporal_store(v, p);
-#else
-  *p = v;
-#endif
-}
-
-/* a new particle (cell word cw, not on an edge) into slot j */
+/* a new particle (cell word cw, not on an edge) into slot j.  Pool
+   accesses are ordinary ones: with non-temporal hints (measured on 8-slot
+   pools) the kernel was 40 % slower, the pools no longer staying in the
+   caches. */
 template <bool COLL>
 __device__ __forceinline__ void put(const pool_ref &h, int j, const mc_state &p, uint32_t id,
                                     uint32_t t0, uint32_t cw) {
-  st_nt(h.seed() + j, p.seed);
-  st_nt(h.x() + j, p.x);
-  st_nt(h.mu() + j, p.mu);
-  st_nt(h.wmc() + j, p.wmc);
-  st_nt(h.cw() + j, cw);
-  st_nt(h.id() + j, id);
-  st_nt(h.t0() + j, t0);
-  if (COLL) st_nt(h.col() + j, 0u);
+  h.seed()[j] = p.seed;
+  h.x()[j] = p.x;
+  h.mu()[j] = p.mu;
+  h.wmc()[j] = p.wmc;
+  h.cw()[j] = cw;
+  h.id()[j] = id;
+  h.t0()[j] = t0;
+  if (COLL) h.col()[j] = 0u;
 }
 
 /* ------------------------------------------------------------ the pipeline
