@@ -335,6 +335,52 @@ def test_event_cap_reports_error(ctx, oracle, kernel):
     assert np.any(result == 2)
 
 
+@pytest.mark.parametrize("cap,ci", [(5, 0), (37, 0), (37, 1), (200, 16)])
+def test_event_cap_same_in_both_kernels(ctx, oracle, cap, ci):
+    """A binding cap on events per history: both kernels stop each longer
+    history after `cap` events, with exit code 2, the same state and the
+    same tally, bit for bit; every history that ends sooner is the
+    oracle's.  The event kernel reads a slot's count origin only while the
+    cap can bind, and moves it at compactions (ci: every pass, every 16th,
+    the default 64th)."""
+    import torch
+    import mcgpu
+    wl = _workload("slab100", 3_000)
+    _say(f"event cap {cap}, compaction interval {ci}")
+    o = _oracle_run(oracle, wl)
+    n = len(wl.bank)
+    got = {}
+    for kernel in KERNELS:
+        bank = mcgpu.bank_to_device(wl.bank)
+        w = torch.zeros(wl.m, dtype=torch.float64, device="cuda")
+        res = torch.empty(n, dtype=torch.int8, device="cuda")
+        ev = torch.empty(n, dtype=torch.int64, device="cuda")
+        co = torch.empty(n, dtype=torch.int64, device="cuda")
+        opts = mcgpu.make_options(kernel, max_events_per_history=cap,
+                                  compaction_interval=ci if kernel == "event" else 0)
+        _, rc = ctx.simulate_device(bank, torch.from_numpy(wl.sigs).cuda(),
+                                    torch.from_numpy(wl.absorption_rates).cuda(), w,
+                                    wl.min_index, wl.max_index, wl.dx, wl.min_weight,
+                                    result=res, events=ev, collisions=co, options=opts,
+                                    allow=(mcgpu.ERR_EVENT_CAP,))
+        got[kernel] = (mcgpu.bank_from_device(bank), res.cpu().numpy().astype(np.int32),
+                       ev.cpu().numpy().view(np.uint64), co.cpu().numpy().view(np.uint64),
+                       w.cpu().numpy(), rc)
+    hb, hr, he, hc, hw, hrc = got["history"]
+    eb, er, ee, ec, ew, erc = got["event"]
+    _assert_bank_equal(eb, hb)
+    assert np.array_equal(er, hr) and np.array_equal(ee, he) and np.array_equal(ec, hc)
+    assert np.array_equal(ew.view(np.uint64), hw.view(np.uint64))
+    longer = o["events"] > cap
+    assert longer.any() and hrc == erc == mcgpu.ERR_EVENT_CAP
+    assert np.all(he[longer] == cap) and np.all(hr[longer] == 2)
+    short = ~longer
+    assert np.array_equal(hr[short], o["result"][short])
+    assert np.array_equal(he[short], o["events"][short])
+    assert np.array_equal(hc[short], o["collisions"][short])
+    _assert_bank_equal(hb[short], o["bank"][short])
+
+
 def test_out_of_range_tally_term(ctx, oracle):
     """A term of 2^39 or more cannot be summed exactly: the cell's tally
     becomes NaN (as the oracle's) and the call reports MCGPU_ERR_RANGE."""
