@@ -87,8 +87,8 @@ typedef struct mcgpu_options {
   int32_t block_size;              /* threads per block, 0 = automatic */
   int32_t grid_blocks;             /* blocks, 0 = automatic */
   int32_t events_per_refill;       /* history kernel: events between refills, 0 = automatic */
-  int32_t compaction_interval;     /* event kernel: passes between compactions, 0 = automatic */
-  int32_t particles_per_lane;      /* event kernel: pool slots per thread, 0 = automatic */
+  int32_t compaction_interval;     /* event kernel: passes between compactions, 0 = automatic (64) */
+  int32_t particles_per_lane;      /* event kernel: pool slots per thread, 0 = automatic (4) */
   uint64_t max_events_per_history; /* guard against runaway histories, 0 = 2^27 */
 } mcgpu_options;
 
