@@ -106,7 +106,12 @@ __device__ __forceinline__ void put(const pool_ref &h, int j, const mc_state &p,
    masked lanes take an offset beyond the pool (no memory access; loads give
    0), and cell and table loads take a valid address on every lane, so each
    stage issues the same memory instructions on every common path: the
-   compiler's wait counts stay exact.  The rare reads (a history's exit, a
+   compiler's wait counts stay exact.  A masked lane still costs the texture
+   address unit its cycles, so the x load and the x and mu stores, which only
+   a collision needs (about one event in a thousand on thin cells), are
+   skipped by a wave-uniform branch when no lane of the chunk needs them; the
+   waits then count for the common path, and the rare one waits a little
+   longer.  The rare reads (a history's exit, a
    compaction pass's id and count origin, a binding cap) sit in branches of
    their own and are waited for inside them.  The passes up to the next
    compaction update the slots in place, so their chunks stream through the
@@ -454,10 +459,12 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
             finish(c_p, id, c_q - t0 + done, c_col);
           }
         }
-        /* the write-back: the offsets per path, then the same stores on every
-           path (a store on one path only would make the compiler assume
-           fewer operations in flight after the branches and wait longer at
-           the next stage); masked lanes aimed beyond the pool */
+        /* the write-back: the offsets per path, then the same stores on both
+           (a store on one path only would make the compiler assume fewer
+           operations in flight after the branches and wait longer at the
+           next stage); masked lanes aimed beyond the pool.  x and mu only
+           when some lane needs them: every compaction pass, else after a
+           collision */
         const uint32_t cw_val =
             keep ? ((uint32_t)(c_p.index - L.index_start) | (on_edge ? ON_EDGE : 0u)) : DEAD_CW;
         const bool w_x = keep && !on_edge; /* x is stored unless implicit */
@@ -479,8 +486,10 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
         }
         const rsrc_t r = compact ? dst : src;
         st64(r, o_seed, 0, c_p.seed);
-        st64(r, o_x, F_X, as_u64(c_p.x));
-        st64(r, o_mu, F_MU, as_u64(c_p.mu));
+        if (compact || __ballot(w_x) != 0ull) {
+          st64(r, o_x, F_X, as_u64(c_p.x));
+          st64(r, o_mu, F_MU, as_u64(c_p.mu));
+        }
         st64(r, o_seed, F_WMC, as_u64(c_p.wmc));
         st32(r, o_cw, F_CW, cw_val);
         if (COLL) st32(r, o_col, F_COL, c_col);
@@ -545,7 +554,8 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
         b_seed = ld64(src, o8, 0);
         b_mu = ld64(src, o8, F_MU);
         b_wmc = ld64(src, o8, F_WMC);
-        b_x = ld64(src, (live && !on_edge) ? 8 * a_slot : OOB, F_X);
+        if (__ballot(live && !on_edge) != 0ull) /* else b_x is not read */
+          b_x = ld64(src, (live && !on_edge) ? 8 * a_slot : OOB, F_X);
         if (COLL) b_col = ld32(src, live ? 4 * a_slot : OOB, F_COL);
         /* the cell, at an index inside the layer on every lane */
         const unsigned cu = a_cw & ~ON_EDGE;
