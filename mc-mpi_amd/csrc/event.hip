@@ -46,19 +46,23 @@ namespace {
 #define ON_EDGE 0x80000000u
 #define DEAD_CW 0xFFFFFFFFu
 
-/* A wave's pool: two halves of P slots each, structure of arrays.  Field
-   k of slot j in half h lives at base + h*HALF + OFF_k*P + j*size_k, with
-   the 8-byte fields first.  pool_ref addresses the fields for the refill;
-   the passes reach them through buffer resources (below). */
+/* A wave's pool: two halves of P slots each, structure of arrays, except
+   that seed and wmc, which every event reads and writes, sit side by side
+   (one 16-byte load and one 16-byte store per slot instead of two of 8:
+   the texture unit's cost goes by instructions).  Field k of slot j in
+   half h lives at base + h*HALF + OFF_k*P + j*size_k: {seed, wmc} at 0,
+   x at 16P, mu at 24P, then the 4-byte cell word, id, count origin and
+   collision count.  pool_ref addresses the fields for the refill; the
+   passes reach them through buffer resources (below). */
 constexpr size_t SLOT_BYTES = 8 * 4 + 4 * 4; /* 48 bytes per slot */
 
 struct pool_ref {
   unsigned char *b; /* base of one half */
   int P;
-  __device__ __forceinline__ uint64_t *seed() const { return reinterpret_cast<uint64_t *>(b); }
-  __device__ __forceinline__ double *x() const { return reinterpret_cast<double *>(b + 8 * (size_t)P); }
-  __device__ __forceinline__ double *mu() const { return reinterpret_cast<double *>(b + 16 * (size_t)P); }
-  __device__ __forceinline__ double *wmc() const { return reinterpret_cast<double *>(b + 24 * (size_t)P); }
+  /* seed and wmc side by side (both change at every event) */
+  __device__ __forceinline__ uint64_t *sw() const { return reinterpret_cast<uint64_t *>(b); }
+  __device__ __forceinline__ double *x() const { return reinterpret_cast<double *>(b + 16 * (size_t)P); }
+  __device__ __forceinline__ double *mu() const { return reinterpret_cast<double *>(b + 24 * (size_t)P); }
   __device__ __forceinline__ uint32_t *cw() const { return reinterpret_cast<uint32_t *>(b + 32 * (size_t)P); }
   __device__ __forceinline__ uint32_t *id() const { return reinterpret_cast<uint32_t *>(b + 36 * (size_t)P); }
   /* the pass at which the particle's event count was 0: every pass gives
@@ -74,10 +78,10 @@ struct pool_ref {
 template <bool COLL>
 __device__ __forceinline__ void put(const pool_ref &h, int j, const mc_state &p, uint32_t id,
                                     uint32_t t0, uint32_t cw) {
-  h.seed()[j] = p.seed;
+  h.sw()[2 * j] = p.seed;
+  h.sw()[2 * j + 1] = __builtin_bit_cast(uint64_t, p.wmc);
   h.x()[j] = p.x;
   h.mu()[j] = p.mu;
-  h.wmc()[j] = p.wmc;
   h.cw()[j] = cw;
   h.id()[j] = id;
   h.t0()[j] = t0;
@@ -131,6 +135,21 @@ __device__ __forceinline__ rsrc_t half_rsrc(unsigned char *half, int P) {
 __device__ __forceinline__ uint64_t ld64(rsrc_t r, int voff, int soff) {
   const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
   return ((uint64_t)v.y << 32) | v.x;
+}
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+/* two 8-byte fields side by side */
+__device__ __forceinline__ void ld128(rsrc_t r, int voff, int soff, uint64_t &a, uint64_t &b) {
+  const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+  a = ((uint64_t)v.y << 32) | v.x;
+  b = ((uint64_t)v.w << 32) | v.z;
+}
+__device__ __forceinline__ void st128(rsrc_t r, int voff, int soff, uint64_t a, uint64_t b) {
+  u32x4 w;
+  w.x = (uint32_t)a;
+  w.y = (uint32_t)(a >> 32);
+  w.z = (uint32_t)b;
+  w.w = (uint32_t)(b >> 32);
+  __builtin_amdgcn_raw_buffer_store_b128(w, r, voff, soff, 0);
 }
 __device__ __forceinline__ uint32_t ld32(rsrc_t r, int voff, int soff) {
   return __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
@@ -310,7 +329,7 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
       own_sreg(a.max_events > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)a.max_events);
   const double dx = L.dx;
   /* field offsets in a half (bytes) */
-  const int F_X = 8 * P, F_MU = 16 * P, F_WMC = 24 * P, F_CW = 32 * P, F_ID = 36 * P,
+  const int F_X = 16 * P, F_MU = 24 * P, F_CW = 32 * P, F_ID = 36 * P,
             F_T0 = 40 * P, F_COL = 44 * P;
   const unsigned ce = (unsigned)compact_every;
 
@@ -473,24 +492,23 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
           const unsigned long long km = __ballot(keep);
           const int pos = wpos + mc_mbcnt64(km);
           wpos += __popcll(km);
-          o_seed = keep ? 8 * pos : OOB;
+          o_seed = keep ? 16 * pos : OOB;
           o_x = w_x ? 8 * pos : OOB;
-          o_mu = o_seed;
+          o_mu = keep ? 8 * pos : OOB;
           o_cw = o_col = keep ? 4 * pos : OOB;
         } else { /* in place: only what the event changed */
-          o_seed = keep ? 8 * c_slot : OOB;
+          o_seed = keep ? 16 * c_slot : OOB;
           o_x = o_mu = w_x ? 8 * c_slot : OOB; /* a collision's x and mu */
           /* a crossing's new cell, a collision's cleared edge bit, a death */
           o_cw = (c_live && (!keep || on_edge || c_was_on_edge)) ? 4 * c_slot : OOB;
           o_col = keep ? 4 * c_slot : OOB;
         }
         const rsrc_t r = compact ? dst : src;
-        st64(r, o_seed, 0, c_p.seed);
+        st128(r, o_seed, 0, c_p.seed, as_u64(c_p.wmc));
         if (compact || __ballot(w_x) != 0ull) {
           st64(r, o_x, F_X, as_u64(c_p.x));
           st64(r, o_mu, F_MU, as_u64(c_p.mu));
         }
-        st64(r, o_seed, F_WMC, as_u64(c_p.wmc));
         st32(r, o_cw, F_CW, cw_val);
         if (COLL) st32(r, o_col, F_COL, c_col);
         if (compact) { /* the id and the count origin move with the slot (the
@@ -550,10 +568,8 @@ __global__ __launch_bounds__(EVENT_MAX_BLOCK) void k_event(mc_transport_args a,
       {
         const bool live = a_valid && a_cw != DEAD_CW;
         const bool on_edge = live && (a_cw & ON_EDGE) != 0;
-        const int o8 = live ? 8 * a_slot : OOB;
-        b_seed = ld64(src, o8, 0);
-        b_mu = ld64(src, o8, F_MU);
-        b_wmc = ld64(src, o8, F_WMC);
+        ld128(src, live ? 16 * a_slot : OOB, 0, b_seed, b_wmc);
+        b_mu = ld64(src, live ? 8 * a_slot : OOB, F_MU);
         if (__ballot(live && !on_edge) != 0ull) /* else b_x is not read */
           b_x = ld64(src, (live && !on_edge) ? 8 * a_slot : OOB, F_X);
         if (COLL) b_col = ld32(src, live ? 4 * a_slot : OOB, F_COL);
