@@ -52,10 +52,10 @@ BYTES_PER_EVENT = 72  # algorithmic: the 36-byte particle state read + written p
 #    an upper bound on HBM bytes, since the Infinity Cache (MALL) in front of
 #    HBM can serve them;
 #  - VALU wave-instructions (SQ_INSTS_VALU).
-MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0616, "event": 21.17}
+MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0616, "event": 19.89}
 # (the event kernel's VALU figure is of a 1e8-particle run, the size bench.py
 # times: at 1e7 the end of the bank, with its partly empty chunks, weighs more)
-MEASURED_VALU_PER_EVENT = {"history": 1.668, "event": 2.844}
+MEASURED_VALU_PER_EVENT = {"history": 1.668, "event": 2.918}
 # gfx950 issues one wave64 VALU instruction per SIMD every 4 cycles:
 # 256 CUs x 4 SIMDs x 2.4 GHz / 4 = 6.144e11 wave-instructions/s.
 VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4
@@ -80,12 +80,13 @@ def roofline(kernel, events, seconds):
       traffic); bound by VALU issue.
     event kernel: every event reloads and stores its particle, but the
       fields a pass touches (about 3.7 MB per XCD for the four-slot pools)
-      stay in L2: about 4 B/event read and 17 B/event written reach the
+      stay in L2: about 3.4 B/event read and 16.5 B/event written reach the
       memory fabric, against the 72 B/event algorithmic yardstick of the
       event formulation (the 36-byte state read and written).  Neither
-      VALU issue (about 70 %) nor HBM (about 40 %) is saturated: the limit
-      is the memory pipeline's per-event requests (pool fields, the cell
-      and exp-table gathers)."""
+      VALU issue (about 74 %) nor HBM (about 40 %) is saturated: the
+      limit is the vector memory pipeline, whose texture address unit is
+      busy about 73 % of the cycles on average and 84 % on the busiest CU
+      (pool fields, the cell and exp-table gathers)."""
     measured_b = MEASURED_TRAFFIC_PER_EVENT.get(kernel)
     traffic = None if measured_b is None else measured_b * events
     alg_gbs = events * BYTES_PER_EVENT / seconds / 1e9
@@ -102,9 +103,10 @@ def roofline(kernel, events, seconds):
     else:
         note = ("event kernel: one event per live particle per pass, state reloaded each event "
                 "from per-wave pools whose touched fields stay in L2; frac = measured "
-                "wave-instructions/event x events/s / 6.144e11, about 0.7: neither VALU issue "
-                "nor HBM (hbm.measured_frac) is saturated, the limit is the memory pipeline's "
-                "per-event requests (pool fields, cell and exp-table gathers); ")
+                "wave-instructions/event x events/s / 6.144e11, about 0.74: neither VALU issue "
+                "nor HBM (hbm.measured_frac) is saturated, the limit is the vector memory "
+                "pipeline (texture address unit busy ~73 % of cycles: pool fields, cell and "
+                "exp-table gathers); ")
     return {"bound": "valu", "achieved": valu["achieved"], "peak": valu["peak"],
             "unit": valu["unit"], "frac": valu["frac"], "traffic": traffic,
             "wave_instructions_per_event": valu["wave_instructions_per_event"],
