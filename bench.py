@@ -55,7 +55,7 @@ BYTES_PER_EVENT = 72  # algorithmic: the 36-byte particle state read + written p
 MEASURED_TRAFFIC_PER_EVENT = {"history": 0.0616, "event": 19.89}
 # (the event kernel's VALU figure is of a 1e8-particle run, the size bench.py
 # times: at 1e7 the end of the bank, with its partly empty chunks, weighs more)
-MEASURED_VALU_PER_EVENT = {"history": 1.668, "event": 2.918}
+MEASURED_VALU_PER_EVENT = {"history": 1.620, "event": 2.918}
 # gfx950 issues one wave64 VALU instruction per SIMD every 4 cycles:
 # 256 CUs x 4 SIMDs x 2.4 GHz / 4 = 6.144e11 wave-instructions/s.
 VALU_PEAK_WAVE_INSTR = 256 * 4 * 2.4e9 / 4

@@ -98,9 +98,8 @@ __device__ __forceinline__ void batch_tile(const batch_args &a, const mc_tile_k 
         const int c = p.index - L.index_start;
         bool coll;
         double di;
-        const double dw = mc_event<true>(p, cells, c, L.dx, K, coll, di);
+        const double dw = mc_event<true>(p, cells, c, L.dx, K, coll, di, nullptr, &col);
         ++ev;
-        col += coll ? 1u : 0u;
         mc_tally<false, HI_COND>(t_lo, t_hi, a.flags + sl.cell_off, &a.ctr->status, c, dw);
       }
       const bool alive = mc_alive(p, L);

@@ -70,6 +70,11 @@ __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refil
      each refill costs a queue atomic and a pass through this loop) */
   const bool adapt = refill_k <= 0;
   int kk = adapt ? 128 : refill_k;
+  /* wave-uniform bound on every lane's event count: while it stays a whole
+     interval below the cap, no history can reach the cap in the next
+     interval, and the events run without the cap test; recomputed exactly
+     (the largest count in the wave) when it nears the cap */
+  uint64_t ev_bound = 0;
   for (;;) {
     if (!done) { /* wave-uniform */
       const unsigned long long need = __ballot(!has);
@@ -99,17 +104,35 @@ __global__ __launch_bounds__(1024) void k_history(mc_transport_args a, int refil
       }
     }
     if (__ballot(has) == 0) break; /* wave-uniform */
+    if (ev_bound + (uint64_t)kk >= a.max_events)
+      ev_bound = mc_wave_max_u64(has ? ev : 0ull);
+    const bool below_cap = ev_bound + (uint64_t)kk < a.max_events; /* wave-uniform */
+    ev_bound += (uint64_t)kk;
     if (has) {
-      for (int k = 0; k < kk; ++k) {
-        if (!mc_alive(p, a.L) || ev >= a.max_events) break;
+      /* one event of this lane's history: Layer::particle_step */
+      auto step = [&]() {
         const int c = p.index - a.L.index_start;
         bool coll;
         double di;
-        const double dw = mc_event<true>(p, cells, c, a.L.dx, K, coll, di);
-        ++ev;
-        col += coll ? 1u : 0u;
+        const double dw = mc_event<true>(p, cells, c, a.L.dx, K, coll, di, nullptr, &col);
         mc_tally(t_lo, t_hi, a.flags, &a.ctr->status, c, dw);
         if (SCORE) score = score + g[c] * dw;
+      };
+      /* the same loop twice, so that the common one carries no cap test
+         (the compiler keeps a shared test in the loop even when the flag
+         guarding it is uniform) */
+      if (below_cap) {
+        for (int k = 0; k < kk; ++k) {
+          if (!mc_alive(p, a.L)) break;
+          step();
+          ++ev;
+        }
+      } else {
+        for (int k = 0; k < kk; ++k) {
+          if (!mc_alive(p, a.L) || ev >= a.max_events) break;
+          step();
+          ++ev;
+        }
       }
       const bool alive = mc_alive(p, a.L);
       if (!alive || ev >= a.max_events) {

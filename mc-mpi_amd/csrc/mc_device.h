@@ -156,13 +156,15 @@ __device__ __forceinline__ mc_coef mc_coef_live() {
    RCP: the edge distance divides by the particle's stored RN(1/mu)
    (mc_div_by_rcp; same result), refreshed when mu changes, for kernels that
    keep the state in registers. */
-template <bool RCP = false>
+template <bool RCP = false, typename Count = uint64_t>
 __device__ __forceinline__ double mc_event(mc_state &p, const mc_cell *cells, int c,
                                            const double dx, const mc_coef &K, bool &collided,
-                                           double &di_out, mc_cell *cell_out = nullptr) {
+                                           double &di_out, mc_cell *cell_out = nullptr,
+                                           Count *col_count = nullptr) {
   /* The edge quotient is formed unconditionally and then selected; when
      |mu| <= EPS the discarded quotient may be inf or NaN and is never used.
-     *cell_out (if given) receives the cell. */
+     *cell_out (if given) receives the cell; *col_count (if given) is
+     incremented on a collision, inside the collision's own branch. */
   /* an unsigned 32-bit byte offset (c is in [0, m)): the load takes the
      uniform base in scalar registers and the offset as is */
   const mc_cell cell =
@@ -194,6 +196,7 @@ __device__ __forceinline__ double mc_event(mc_state &p, const mc_cell *cells, in
     p.x += di * p.mu;
     p.mu = 2 * mc_rnd_real(p.seed) - 1;
     if (RCP) mc_refresh_rcp(p);
+    if (col_count) ++*col_count;
   } else {
     p.x = x_new_edge;
   }
@@ -271,6 +274,16 @@ __device__ __forceinline__ unsigned long long mc_readlane64(unsigned long long v
 __device__ __forceinline__ int mc_mbcnt64(unsigned long long mask) {
   return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
                                         __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+/* the largest v over the wave, as a wave-uniform (scalar) value; every lane
+   of the wave must take part */
+__device__ __forceinline__ unsigned long long mc_wave_max_u64(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(v, off);
+    v = o > v ? o : v;
+  }
+  return mc_readlane64(v, 0);
 }
 
 __device__ __forceinline__ unsigned long long mc_wave_sum(unsigned long long v) {
