@@ -115,9 +115,9 @@ __device__ __forceinline__ void put(const pool_ref &h, int j, const mc_state &p,
    a collision needs (about one event in a thousand on thin cells), are
    skipped by a wave-uniform branch when no lane of the chunk needs them; the
    waits then count for the common path, and the rare one waits a little
-   longer.  The rare reads (a history's exit, a
-   compaction pass's id and count origin, a binding cap) sit in branches of
-   their own and are waited for inside them.  The passes up to the next
+   longer.  The rare reads (a history's exit, a compaction pass's id and
+   count origin, a binding cap) sit in branches of their own and are waited
+   for inside them.  The passes up to the next
    compaction update the slots in place, so their chunks stream through the
    pipeline without a break (a slot is written in stage D of one pass before
    stage A of the next pass reads it, given at least four chunks); the
