@@ -335,14 +335,18 @@ def test_event_cap_reports_error(ctx, oracle, kernel):
     assert np.any(result == 2)
 
 
-@pytest.mark.parametrize("cap,ci", [(5, 0), (37, 0), (37, 1), (200, 16)])
-def test_event_cap_same_in_both_kernels(ctx, oracle, cap, ci):
+@pytest.mark.parametrize("cap,ci,refill", [(5, 0, 0), (37, 0, 0), (37, 1, 0), (200, 16, 0),
+                                           (250, 0, 64)])
+def test_event_cap_same_in_both_kernels(ctx, oracle, cap, ci, refill):
     """A binding cap on events per history: both kernels stop each longer
     history after `cap` events, with exit code 2, the same state and the
     same tally, bit for bit; every history that ends sooner is the
     oracle's.  The event kernel reads a slot's count origin only while the
     cap can bind, and moves it at compactions (ci: every pass, every 16th,
-    the default 64th)."""
+    the default 64th).  The history kernel drops the per-event cap test
+    while its wave's counts are an interval below the cap: with 64 events
+    between refills and a cap of 250 it runs three intervals that way, then
+    tests every event."""
     import torch
     import mcgpu
     wl = _workload("slab100", 3_000)
@@ -357,7 +361,8 @@ def test_event_cap_same_in_both_kernels(ctx, oracle, cap, ci):
         ev = torch.empty(n, dtype=torch.int64, device="cuda")
         co = torch.empty(n, dtype=torch.int64, device="cuda")
         opts = mcgpu.make_options(kernel, max_events_per_history=cap,
-                                  compaction_interval=ci if kernel == "event" else 0)
+                                  compaction_interval=ci if kernel == "event" else 0,
+                                  events_per_refill=refill if kernel == "history" else 0)
         _, rc = ctx.simulate_device(bank, torch.from_numpy(wl.sigs).cuda(),
                                     torch.from_numpy(wl.absorption_rates).cuda(), w,
                                     wl.min_index, wl.max_index, wl.dx, wl.min_weight,
