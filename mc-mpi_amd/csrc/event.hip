@@ -9,9 +9,10 @@
  *
  * Layout and schedule (persistent blocks, no launch per pass):
  *  - each wave owns a particle pool in device memory: two halves (ping-pong)
- *    of P = 64 * particles_per_lane slots in structure-of-arrays form (seed,
- *    x, mu, wmc, cell word, id, the pass its event count started from,
- *    collision counter), so every field is a coalesced wave access;
+ *    of P = 64 * particles_per_lane slots in structure-of-arrays form
+ *    ({seed, wmc} side by side, x, mu, cell word, id, the pass its event
+ *    count started from, collision counter), so every field is a coalesced
+ *    wave access;
  *  - a pass gives every live slot one event, 64 slots (a chunk) at a time,
  *    through a four-stage software pipeline (below); the survivors of a
  *    compaction pass are packed into the other half -- a wave ballot +
